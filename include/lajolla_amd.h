@@ -1,0 +1,355 @@
+/*
+ * lajolla_amd.h -- C ABI of the MI355X (gfx950) path-tracing core.
+ *
+ * The reference renderer (lajolla) exposes three C++ entry points to its
+ * callers and one native dependency under its hot path:
+ *
+ *   std::unique_ptr<Scene> parse_scene(const fs::path&, const RTCDevice&)
+ *                                                  src/parsers/parse_scene.h:9
+ *   Image3 render(const Scene&)                    src/render.h:9
+ *   std::optional<PathVertex> intersect(const Scene&, const Ray&, ...)
+ *   bool occluded(const Scene&, const Ray&)        src/intersection.h:39-44
+ *     (both sit on Embree's rtcIntersect1 / rtcOccluded1,
+ *      embree/include/embree4/rtcore_scene.h:160,198)
+ *
+ * This header replaces them with plain-pointer entry points.  Closed
+ * std::variant types (Material src/material.h:102-110, Shape src/shape.h:53,
+ * Light src/light.h:34, Medium src/medium.h:22, PhaseFunction
+ * src/phase_function.h:16, Filter src/filter.h:38) become {tag, payload}
+ * POD records.  Every call returns an int status (LJX_OK == 0); no C++
+ * exception crosses the boundary, and ljx_last_error() gives the message of
+ * the last failure on the calling thread.
+ *
+ * Ownership: the caller owns every input array; ljx_scene_create deep-copies
+ * the descriptor into device memory and keeps no pointer into caller memory.
+ * A scene handle owns its device buffers, BVH and path queues.  Calls on one
+ * handle must be externally synchronised; different handles may be used from
+ * different threads.
+ */
+#ifndef LAJOLLA_AMD_H
+#define LAJOLLA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LJX_ABI_VERSION 1
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+    LJX_OK = 0,
+    LJX_ERR_INVALID_ARGUMENT = 1, /* malformed descriptor or argument */
+    LJX_ERR_OUT_OF_MEMORY = 2,    /* device or host allocation failed */
+    LJX_ERR_HIP = 3,              /* a HIP runtime call failed */
+    LJX_ERR_QUEUE_OVERFLOW = 4,   /* a path queue exceeded its capacity */
+    LJX_ERR_UNSUPPORTED = 5,      /* a feature outside this build's scope */
+    LJX_ERR_NO_DEVICE = 6         /* no usable gfx950 device */
+};
+
+/* ---- variant tags (same order as the reference's std::variant lists) ----- */
+/* Shape = variant<Sphere, TriangleMesh>                  src/shape.h:53 */
+enum { LJX_SHAPE_SPHERE = 0, LJX_SHAPE_TRIANGLE_MESH = 1 };
+
+/* Material = variant<Lambertian, RoughPlastic, RoughDielectric, DisneyDiffuse,
+ * DisneyMetal, DisneyGlass, DisneyClearcoat, DisneySheen, DisneyBSDF>
+ *                                                        src/material.h:102-110 */
+enum {
+    LJX_MAT_LAMBERTIAN = 0,
+    LJX_MAT_ROUGH_PLASTIC = 1,
+    LJX_MAT_ROUGH_DIELECTRIC = 2,
+    LJX_MAT_DISNEY_DIFFUSE = 3,
+    LJX_MAT_DISNEY_METAL = 4,
+    LJX_MAT_DISNEY_GLASS = 5,
+    LJX_MAT_DISNEY_CLEARCOAT = 6,
+    LJX_MAT_DISNEY_SHEEN = 7,
+    LJX_MAT_DISNEY_BSDF = 8,
+    LJX_MAT_COUNT = 9
+};
+
+/* Light = variant<DiffuseAreaLight, Envmap>              src/light.h:34 */
+enum { LJX_LIGHT_DIFFUSE_AREA = 0, LJX_LIGHT_ENVMAP = 1 };
+
+/* Medium = variant<HomogeneousMedium, HeterogeneousMedium> src/medium.h:22 */
+enum { LJX_MEDIUM_HOMOGENEOUS = 0, LJX_MEDIUM_HETEROGENEOUS = 1 };
+
+/* PhaseFunction = variant<IsotropicPhase, HenyeyGreenstein> src/phase_function.h:16 */
+enum { LJX_PHASE_ISOTROPIC = 0, LJX_PHASE_HENYEY_GREENSTEIN = 1 };
+
+/* Filter = variant<Box, Tent, Gaussian>                  src/filter.h:38 */
+enum { LJX_FILTER_BOX = 0, LJX_FILTER_TENT = 1, LJX_FILTER_GAUSSIAN = 2 };
+
+/* Integrator                                             src/scene.h:14-22 */
+enum {
+    LJX_INTEGRATOR_DEPTH = 0,
+    LJX_INTEGRATOR_SHADING_NORMAL = 1,
+    LJX_INTEGRATOR_MEAN_CURVATURE = 2,
+    LJX_INTEGRATOR_RAY_DIFFERENTIAL = 3,
+    LJX_INTEGRATOR_MIPMAP_LEVEL = 4,
+    LJX_INTEGRATOR_PATH = 5,
+    LJX_INTEGRATOR_VOLPATH = 6
+};
+
+/* ---- scene descriptor ---------------------------------------------------- */
+
+/* One shape (src/shape.h:20-51).  Triangle meshes point at caller arrays;
+ * positions/normals are xyz per vertex, uvs are uv per vertex, indices are
+ * three int32 per triangle.  normals and uvs may be NULL. */
+typedef struct ljx_shape_desc {
+    uint32_t type;                 /* LJX_SHAPE_* */
+    int32_t material_id;           /* -1: index-matched interface (volpath) */
+    int32_t area_light_id;         /* -1: not emissive */
+    int32_t interior_medium_id;    /* -1: vacuum */
+    int32_t exterior_medium_id;
+    /* triangle mesh */
+    const float *positions;
+    const int32_t *indices;
+    const float *normals;
+    const float *uvs;
+    uint64_t num_vertices;
+    uint64_t num_triangles;
+    /* sphere */
+    double center[3];
+    double radius;
+} ljx_shape_desc;
+
+/* One material with constant textures (src/material.h:12-98).  Field use
+ * per tag:
+ *   LAMBERTIAN        color_a = reflectance
+ *   ROUGH_PLASTIC     color_a = diffuse_reflectance, color_b = specular_reflectance,
+ *                     roughness, eta
+ *   ROUGH_DIELECTRIC  color_a = specular_reflectance, color_b = specular_transmittance,
+ *                     roughness, eta
+ *   DISNEY_*          color_a = base_color, the scalar Disney parameters, eta
+ */
+typedef struct ljx_material_desc {
+    uint32_t type;                 /* LJX_MAT_* */
+    double color_a[3];
+    double color_b[3];
+    double roughness;
+    double eta;
+    double specular_transmission;
+    double metallic;
+    double subsurface;
+    double specular;
+    double specular_tint;
+    double anisotropic;
+    double sheen;
+    double sheen_tint;
+    double clearcoat;
+    double clearcoat_gloss;
+} ljx_material_desc;
+
+/* One light (src/light.h:12-30).  An environment map is an image of
+ * env_width x env_height RGB texels (row-major, float); this build accepts
+ * the 1x1 case, which is how a constant environment is expressed. */
+typedef struct ljx_light_desc {
+    uint32_t type;                 /* LJX_LIGHT_* */
+    int32_t shape_id;              /* diffuse area light: emitting shape */
+    double intensity[3];           /* diffuse area light radiance */
+    const float *env_rgb;          /* envmap texels */
+    int32_t env_width;
+    int32_t env_height;
+    double to_world[16];           /* envmap frame, row-major 4x4 */
+    double scale;                  /* envmap radiance scale */
+} ljx_light_desc;
+
+/* One participating medium (src/medium.h:10-22). */
+typedef struct ljx_medium_desc {
+    uint32_t type;                 /* LJX_MEDIUM_* (homogeneous only here) */
+    double sigma_a[3];
+    double sigma_s[3];
+    uint32_t phase_type;           /* LJX_PHASE_* */
+    double g;                      /* Henyey-Greenstein asymmetry */
+} ljx_medium_desc;
+
+/* Pinhole camera (src/camera.h:7-22).  fov is the x-axis field of view in
+ * degrees, after the parser's fovAxis conversion (parse_scene.cpp:830-843). */
+typedef struct ljx_camera_desc {
+    double cam_to_world[16];       /* row-major 4x4 */
+    double fov;
+    int32_t width;
+    int32_t height;
+    uint32_t filter_type;          /* LJX_FILTER_* */
+    double filter_param;           /* box/tent width, gaussian stddev */
+    int32_t medium_id;             /* -1: camera in vacuum */
+} ljx_camera_desc;
+
+/* RenderOptions (src/scene.h:24-31) plus the sample seed. */
+typedef struct ljx_render_options {
+    uint32_t integrator;           /* LJX_INTEGRATOR_* */
+    int32_t samples_per_pixel;
+    int32_t max_depth;             /* -1: Russian roulette only */
+    int32_t rr_depth;
+    int32_t vol_path_version;
+    int32_t max_null_collisions;
+    uint64_t seed;                 /* mixed into every (pixel, sample) stream */
+} ljx_render_options;
+
+typedef struct ljx_scene_desc {
+    ljx_camera_desc camera;
+    const ljx_shape_desc *shapes;
+    int32_t num_shapes;
+    const ljx_material_desc *materials;
+    int32_t num_materials;
+    const ljx_light_desc *lights;
+    int32_t num_lights;
+    const ljx_medium_desc *media;
+    int32_t num_media;
+    int32_t envmap_light_id;       /* -1 if none (src/scene.h:66) */
+    ljx_render_options options;
+} ljx_scene_desc;
+
+/* ---- rays and hits for the intersection boundary ------------------------ */
+
+/* A ray in FP32, as Embree receives it (src/intersection.cpp:15-31). */
+typedef struct ljx_ray {
+    float org[3];
+    float tnear;
+    float dir[3];
+    float tfar;
+} ljx_ray;
+
+/* Closest hit: shape_id == -1 for a miss.  u weights vertex 1 and v weights
+ * vertex 2, as Embree's RTCHit does (src/intersection.cpp:48). */
+typedef struct ljx_hit {
+    float t;
+    float u;
+    float v;
+    int32_t shape_id;
+    int32_t prim_id;
+    float ng[3];                   /* unnormalised geometric normal */
+} ljx_hit;
+
+/* ---- BSDF evaluation boundary (material-level parity) ------------------- */
+
+/* One query of eval / pdf_sample_bsdf / sample_bsdf (src/material.h:126-166)
+ * at a shading point with the given frame.  Directions point away from the
+ * surface.  All FP64, like the reference's Real. */
+typedef struct ljx_bsdf_query {
+    int32_t material_id;
+    int32_t transport_to_view;     /* 0: TO_LIGHT (default), 1: TO_VIEW */
+    double dir_in[3];
+    double dir_out[3];
+    double geometric_normal[3];
+    double frame_x[3];
+    double frame_y[3];
+    double frame_n[3];
+    double rnd_uv[2];
+    double rnd_w;
+} ljx_bsdf_query;
+
+typedef struct ljx_bsdf_result {
+    double f[3];                   /* eval(dir_in, dir_out) incl. cosine */
+    double pdf;                    /* pdf_sample_bsdf(dir_in, dir_out) */
+    double sample_dir[3];          /* sample_bsdf(dir_in, rnd) */
+    double sample_eta;
+    double sample_roughness;
+    int32_t sample_valid;          /* 0 where the reference returns nullopt */
+} ljx_bsdf_result;
+
+/* ---- render statistics -------------------------------------------------- */
+typedef struct ljx_render_stats {
+    uint64_t camera_paths;         /* W*H*spp */
+    uint64_t extension_rays;       /* closest-hit queries traced */
+    uint64_t shadow_rays;          /* any-hit queries traced */
+    uint64_t passes;               /* wavefront passes launched */
+    double total_ms;               /* wall time of the render call */
+    double extend_ms;              /* HIP-event time in the closest-hit kernel */
+    uint64_t extend_launches;
+    double shadow_ms;              /* HIP-event time in the any-hit kernel */
+    uint64_t shadow_launches;
+    double shade_ms;               /* HIP-event time in the shading kernels */
+    double other_ms;               /* raygen, accumulate, bookkeeping kernels */
+    uint64_t node_visits;          /* filled only by instrumented runs */
+    uint64_t tri_tests;
+    uint64_t instrumented_rays;
+} ljx_render_stats;
+
+/* Render-call switches that do not change results. */
+typedef struct ljx_exec_options {
+    int32_t collect_timing;        /* 1: HIP events around every kernel */
+    int32_t collect_traversal;     /* 1: count node visits / tri tests */
+    int32_t max_paths_in_flight;   /* 0: library default */
+    int32_t block_size;            /* 0: library default (multiple of 64) */
+    int32_t sort_threshold;        /* 0: library default; <0: never sort */
+} ljx_exec_options;
+
+/* ---- entry points -------------------------------------------------------- */
+
+/* Library ABI version (LJX_ABI_VERSION).  No reference counterpart. */
+int ljx_abi_version(void);
+
+/* Copies the message of the last failed call on this thread (NUL-terminated,
+ * truncated to cap bytes).  Replaces fl_exception::what() (src/flexception.h:8-24). */
+int ljx_last_error(char *buf, size_t cap);
+
+/* Number of usable gfx950 devices.  Replaces rtcNewDevice (src/main.cpp:30). */
+int ljx_device_count(int *count);
+
+/* Builds the device scene: uploads geometry, builds the SAH BVH, light and
+ * triangle sampling tables and the scene bounding sphere.  Replaces
+ * Scene::Scene (src/scene.cpp:4-53) including rtcCommitScene. */
+int ljx_scene_create(const ljx_scene_desc *desc, int device, void **scene_out);
+
+/* Frees every device buffer of the scene.  Replaces Scene::~Scene
+ * (src/scene.cpp:55-59). */
+int ljx_scene_destroy(void *scene);
+
+/* Scene bounding sphere (src/scene.cpp:30-34): center xyz, radius. */
+int ljx_scene_bounds(void *scene, double *center_radius4);
+
+/* Renders into rgb_out (host memory, height*width*3 floats, HWC, the pixel
+ * mean over samples_per_pixel).  Replaces render() (src/render.cpp:155-170).
+ * opts may be NULL (scene options are used); exec and stats may be NULL. */
+int ljx_render(void *scene, const ljx_render_options *opts,
+               const ljx_exec_options *exec, float *rgb_out,
+               ljx_render_stats *stats);
+
+/* Same as ljx_render, but rgb_out is device memory on the scene's device;
+ * no host copy is made.  For callers that keep the image in HBM. */
+int ljx_render_device(void *scene, const ljx_render_options *opts,
+                      const ljx_exec_options *exec, float *rgb_out_device,
+                      ljx_render_stats *stats);
+
+/* Closest hit for n rays in host memory.  Replaces rtcIntersect1 as called
+ * by intersect() (src/intersection.cpp:7-33). */
+int ljx_trace_closest(void *scene, const ljx_ray *rays, size_t n,
+                      ljx_hit *hits);
+
+/* Any hit for n rays in host memory; occluded[i] = 1 if blocked within
+ * [tnear, tfar].  Replaces rtcOccluded1 (src/intersection.cpp:67-85). */
+int ljx_trace_any(void *scene, const ljx_ray *rays, size_t n,
+                  uint8_t *occluded);
+
+/* Device-memory variants of the two calls above (rays, hits and flags are
+ * device pointers on the scene's device).  elapsed_ms (may be NULL) gets the
+ * HIP-event time of the traversal kernel alone; stats (may be NULL) counts
+ * node visits and triangle tests when exec->collect_traversal is set. */
+int ljx_trace_closest_device(void *scene, const ljx_ray *rays, size_t n,
+                             ljx_hit *hits, const ljx_exec_options *exec,
+                             double *elapsed_ms, ljx_render_stats *stats);
+int ljx_trace_any_device(void *scene, const ljx_ray *rays, size_t n,
+                         uint8_t *occluded, const ljx_exec_options *exec,
+                         double *elapsed_ms, ljx_render_stats *stats);
+
+/* Evaluates eval / pdf_sample_bsdf / sample_bsdf of the scene's materials on
+ * the device for n queries in host memory, with the same device code the
+ * shading kernels run.  Replaces the three dispatchers in
+ * src/material.cpp:90-119. */
+int ljx_bsdf_batch(void *scene, const ljx_bsdf_query *queries, size_t n,
+                   ljx_bsdf_result *results);
+
+/* Device memory helpers for callers without their own allocator. */
+int ljx_device_alloc(int device, size_t bytes, void **ptr);
+int ljx_device_free(void *ptr);
+int ljx_copy_to_device(void *dst, const void *src, size_t bytes);
+int ljx_copy_to_host(void *dst, const void *src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LAJOLLA_AMD_H */

@@ -1,0 +1,488 @@
+"""Scene descriptions and seeded synthetic scene generators.
+
+A :class:`SceneDesc` holds numpy arrays and plain Python records shaped like
+lajolla's Scene (src/scene.h:42-82): a camera, materials, shapes, lights,
+media, the environment-map light id and RenderOptions.  ``to_c()`` packs it
+into the ``ljx_scene_desc`` C struct of include/lajolla_amd.h.
+
+The generators build the workloads the benchmarks and tests use, each from a
+recorded 64-bit seed:
+
+* ``cornell_box``      -- 256x256, 16 spp, depth 6, 36 triangles, Lambertian
+* ``terrain_spheres``  -- 1920x1080, 64 spp, depth 8, ~2.31M triangles,
+                          Lambertian / RoughPlastic / DisneyBSDF, 1024
+                          emissive triangles and a constant environment
+* ``nested_media``     -- 1024x1024, 128 spp, depth 16, homogeneous media
+                          inside a 100k-triangle sphere nested in a
+                          dielectric sphere, Henyey-Greenstein phase
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _capi as capi
+
+
+# ----------------------------------------------------------------- transforms
+def normalize(v):
+    v = np.asarray(v, dtype=np.float64)
+    n = np.linalg.norm(v)
+    return v / n if n > 0 else np.zeros(3)
+
+
+def look_at(pos, look, up):
+    """Camera-to-world matrix, same construction as src/transform.cpp:46-70."""
+    pos = np.asarray(pos, np.float64)
+    d = normalize(np.asarray(look, np.float64) - pos)
+    left = normalize(np.cross(normalize(up), d))
+    new_up = np.cross(d, left)
+    m = np.eye(4)
+    m[:3, 0] = left
+    m[:3, 1] = new_up
+    m[:3, 2] = d
+    m[:3, 3] = pos
+    return m
+
+
+# ------------------------------------------------------------------ the desc
+class SceneDesc:
+    def __init__(self):
+        self.camera = dict(cam_to_world=np.eye(4), fov=45.0, width=64, height=64,
+                           filter_type=capi.FILTER_BOX, filter_param=1.0, medium_id=-1)
+        self.materials = []
+        self.shapes = []
+        self.lights = []
+        self.media = []
+        self.envmap_light_id = -1
+        self.options = dict(integrator=capi.INTEGRATOR_PATH, samples_per_pixel=4,
+                            max_depth=-1, rr_depth=5, vol_path_version=0,
+                            max_null_collisions=1000, seed=capi.DEFAULT_SEED)
+        self.name = "scene"
+        self.meta = {}
+
+    # -- builders
+    def add_material(self, type, color_a=(0, 0, 0), color_b=(0, 0, 0), roughness=0.0, eta=1.5, **kw):
+        m = dict(type=type, color_a=tuple(color_a), color_b=tuple(color_b),
+                 roughness=roughness, eta=eta)
+        for k in ("specular_transmission", "metallic", "subsurface", "specular", "specular_tint",
+                  "anisotropic", "sheen", "sheen_tint", "clearcoat", "clearcoat_gloss"):
+            m[k] = kw.get(k, 0.0)
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def add_mesh(self, positions, indices, material_id, normals=None, uvs=None,
+                 interior_medium_id=-1, exterior_medium_id=-1):
+        s = dict(type=capi.SHAPE_TRIANGLE_MESH, material_id=material_id, area_light_id=-1,
+                 interior_medium_id=interior_medium_id, exterior_medium_id=exterior_medium_id,
+                 positions=np.ascontiguousarray(positions, np.float32).reshape(-1, 3),
+                 indices=np.ascontiguousarray(indices, np.int32).reshape(-1, 3),
+                 normals=None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3),
+                 uvs=None if uvs is None else np.ascontiguousarray(uvs, np.float32).reshape(-1, 2),
+                 center=(0.0, 0.0, 0.0), radius=0.0)
+        self.shapes.append(s)
+        return len(self.shapes) - 1
+
+    def add_sphere(self, center, radius, material_id, interior_medium_id=-1, exterior_medium_id=-1):
+        s = dict(type=capi.SHAPE_SPHERE, material_id=material_id, area_light_id=-1,
+                 interior_medium_id=interior_medium_id, exterior_medium_id=exterior_medium_id,
+                 positions=None, indices=None, normals=None, uvs=None,
+                 center=tuple(float(c) for c in center), radius=float(radius))
+        self.shapes.append(s)
+        return len(self.shapes) - 1
+
+    def add_area_light(self, shape_id, intensity):
+        self.lights.append(dict(type=capi.LIGHT_DIFFUSE_AREA, shape_id=shape_id,
+                                intensity=tuple(intensity)))
+        self.shapes[shape_id]["area_light_id"] = len(self.lights) - 1
+        return len(self.lights) - 1
+
+    def add_constant_envmap(self, rgb, scale=1.0, to_world=None):
+        self.lights.append(dict(type=capi.LIGHT_ENVMAP, shape_id=-1, intensity=(0, 0, 0),
+                                env=np.asarray(rgb, np.float32).reshape(1, 1, 3),
+                                to_world=np.eye(4) if to_world is None else np.asarray(to_world),
+                                scale=scale))
+        self.envmap_light_id = len(self.lights) - 1
+        return self.envmap_light_id
+
+    def add_medium(self, sigma_a, sigma_s, phase_type=capi.PHASE_ISOTROPIC, g=0.0):
+        self.media.append(dict(type=capi.MEDIUM_HOMOGENEOUS, sigma_a=tuple(sigma_a),
+                               sigma_s=tuple(sigma_s), phase_type=phase_type, g=g))
+        return len(self.media) - 1
+
+    @property
+    def num_triangles(self):
+        return int(sum(len(s["indices"]) for s in self.shapes if s["type"] == capi.SHAPE_TRIANGLE_MESH))
+
+    # -- C packing
+    def to_c(self):
+        keep = []
+        d = capi.SceneDescC()
+        cam = self.camera
+        d.camera.cam_to_world = capi.D16(*np.asarray(cam["cam_to_world"], np.float64).reshape(16))
+        d.camera.fov = cam["fov"]
+        d.camera.width = cam["width"]
+        d.camera.height = cam["height"]
+        d.camera.filter_type = cam["filter_type"]
+        d.camera.filter_param = cam["filter_param"]
+        d.camera.medium_id = cam["medium_id"]
+
+        def ptr(a, ctype):
+            if a is None:
+                return C.POINTER(ctype)()
+            a = np.ascontiguousarray(a)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(ctype))
+
+        shapes = (capi.ShapeDesc * max(1, len(self.shapes)))()
+        for i, s in enumerate(self.shapes):
+            o = shapes[i]
+            o.type = s["type"]
+            o.material_id = s["material_id"]
+            o.area_light_id = s["area_light_id"]
+            o.interior_medium_id = s["interior_medium_id"]
+            o.exterior_medium_id = s["exterior_medium_id"]
+            if s["type"] == capi.SHAPE_TRIANGLE_MESH:
+                o.positions = ptr(s["positions"], C.c_float)
+                o.indices = ptr(s["indices"], C.c_int32)
+                o.normals = ptr(s["normals"], C.c_float)
+                o.uvs = ptr(s["uvs"], C.c_float)
+                o.num_vertices = len(s["positions"])
+                o.num_triangles = len(s["indices"])
+            o.center = capi.D3(*s["center"])
+            o.radius = s["radius"]
+        mats = (capi.MaterialDesc * max(1, len(self.materials)))()
+        for i, m in enumerate(self.materials):
+            o = mats[i]
+            o.type = m["type"]
+            o.color_a = capi.D3(*m["color_a"])
+            o.color_b = capi.D3(*m["color_b"])
+            for k in ("roughness", "eta", "specular_transmission", "metallic", "subsurface", "specular",
+                      "specular_tint", "anisotropic", "sheen", "sheen_tint", "clearcoat", "clearcoat_gloss"):
+                setattr(o, k, float(m[k]))
+        lights = (capi.LightDesc * max(1, len(self.lights)))()
+        for i, l in enumerate(self.lights):
+            o = lights[i]
+            o.type = l["type"]
+            o.shape_id = l["shape_id"]
+            o.intensity = capi.D3(*l["intensity"])
+            if l["type"] == capi.LIGHT_ENVMAP:
+                o.env_rgb = ptr(l["env"], C.c_float)
+                o.env_height, o.env_width = l["env"].shape[:2]
+                o.to_world = capi.D16(*np.asarray(l["to_world"], np.float64).reshape(16))
+                o.scale = l["scale"]
+        media = (capi.MediumDesc * max(1, len(self.media)))()
+        for i, m in enumerate(self.media):
+            o = media[i]
+            o.type = m["type"]
+            o.sigma_a = capi.D3(*m["sigma_a"])
+            o.sigma_s = capi.D3(*m["sigma_s"])
+            o.phase_type = m["phase_type"]
+            o.g = m["g"]
+        keep += [shapes, mats, lights, media]
+        d.shapes = C.cast(shapes, C.POINTER(capi.ShapeDesc))
+        d.num_shapes = len(self.shapes)
+        d.materials = C.cast(mats, C.POINTER(capi.MaterialDesc))
+        d.num_materials = len(self.materials)
+        d.lights = C.cast(lights, C.POINTER(capi.LightDesc))
+        d.num_lights = len(self.lights)
+        d.media = C.cast(media, C.POINTER(capi.MediumDesc))
+        d.num_media = len(self.media)
+        d.envmap_light_id = self.envmap_light_id
+        d.options = render_options(**self.options)
+        return d, keep
+
+
+def render_options(integrator=capi.INTEGRATOR_PATH, samples_per_pixel=4, max_depth=-1, rr_depth=5,
+                   vol_path_version=0, max_null_collisions=1000, seed=capi.DEFAULT_SEED):
+    o = capi.RenderOptions()
+    o.integrator = integrator
+    o.samples_per_pixel = samples_per_pixel
+    o.max_depth = max_depth
+    o.rr_depth = rr_depth
+    o.vol_path_version = vol_path_version
+    o.max_null_collisions = max_null_collisions
+    o.seed = seed
+    return o
+
+
+# --------------------------------------------------------------- geometry kit
+def quad(p0, p1, p2, p3):
+    """Two triangles (p0,p1,p2), (p0,p2,p3); the normal is (p1-p0)x(p2-p0)."""
+    P = np.array([p0, p1, p2, p3], np.float32)
+    I = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    n = np.cross(P[1] - P[0], P[2] - P[0]).astype(np.float64)
+    n /= np.linalg.norm(n)
+    N = np.tile(n.astype(np.float32), (4, 1))
+    return P, I, N
+
+
+def box(center, size, angle_deg):
+    """Axis-aligned box rotated about +y; 6 faces with outward flat normals."""
+    cx, cy, cz = center
+    sx, sy, sz = [s / 2 for s in size]
+    a = math.radians(angle_deg)
+    ca, sa = math.cos(a), math.sin(a)
+
+    def xf(x, y, z):
+        return (cx + ca * x + sa * z, cy + y, cz - sa * x + ca * z)
+
+    c = [(-sx, -sy, -sz), (sx, -sy, -sz), (sx, sy, -sz), (-sx, sy, -sz),
+         (-sx, -sy, sz), (sx, -sy, sz), (sx, sy, sz), (-sx, sy, sz)]
+    faces = [(4, 5, 6, 7), (1, 0, 3, 2), (0, 4, 7, 3), (5, 1, 2, 6), (7, 6, 2, 3), (0, 1, 5, 4)]
+    Ps, Is, Ns = [], [], []
+    for k, f in enumerate(faces):
+        P, I, N = quad(*[xf(*c[j]) for j in f])
+        Ps.append(P)
+        Is.append(I + 4 * k)
+        Ns.append(N)
+    return np.concatenate(Ps), np.concatenate(Is), np.concatenate(Ns)
+
+
+def icosphere(subdiv):
+    """Unit icosphere by midpoint subdivision; outward CCW winding."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+         (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    V = [tuple(np.array(v) / np.linalg.norm(v)) for v in V]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+         (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            k = (a, b) if a < b else (b, a)
+            if k not in cache:
+                m = np.add(V[a], V[b]) / 2
+                V.append(tuple(m / np.linalg.norm(m)))
+                cache[k] = len(V) - 1
+            return cache[k]
+
+        for a, b, c in F:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        F = nf
+    P = np.array(V, np.float64)
+    I = np.array(F, np.int32)
+    # orient outward
+    n = np.cross(P[I[:, 1]] - P[I[:, 0]], P[I[:, 2]] - P[I[:, 0]])
+    flip = np.einsum("ij,ij->i", n, P[I].mean(axis=1)) < 0
+    I[flip] = I[flip][:, [0, 2, 1]]
+    return P, I
+
+
+def uv_sphere(n_theta, n_phi):
+    """UV sphere with n_theta*n_phi*2 triangles (poles use degenerate-free fans
+    by clamping the ring count); outward winding and smooth normals."""
+    th = np.linspace(0, np.pi, n_theta + 1)
+    ph = np.linspace(0, 2 * np.pi, n_phi + 1)[:-1]
+    T, Ph = np.meshgrid(th, ph, indexing="ij")
+    P = np.stack([np.sin(T) * np.cos(Ph), np.cos(T), np.sin(T) * np.sin(Ph)], -1).reshape(-1, 3)
+    idx = lambda i, j: i * n_phi + (j % n_phi)
+    I = []
+    for i in range(n_theta):
+        for j in range(n_phi):
+            a, b, c, d = idx(i, j), idx(i, j + 1), idx(i + 1, j + 1), idx(i + 1, j)
+            I.append((a, b, c))
+            I.append((a, c, d))
+    I = np.array(I, np.int32)
+    n = np.cross(P[I[:, 1]] - P[I[:, 0]], P[I[:, 2]] - P[I[:, 0]])
+    flip = np.einsum("ij,ij->i", n, P[I].mean(axis=1)) < 0
+    I[flip] = I[flip][:, [0, 2, 1]]
+    return P, I
+
+
+def value_noise_fbm(x, z, rng, octaves=6, lacunarity=2.0, gain=0.5, base_freq=0.15):
+    """Seeded fBm of value noise with smoothstep interpolation."""
+    perm = rng.permutation(256)
+    vals = rng.uniform(-1, 1, 256)
+
+    def lattice(ix, iz):
+        return vals[perm[(perm[ix & 255] + iz) & 255]]
+
+    h = np.zeros_like(x)
+    amp, freq = 1.0, base_freq
+    for _ in range(octaves):
+        fx, fz = x * freq, z * freq
+        ix, iz = np.floor(fx).astype(np.int64), np.floor(fz).astype(np.int64)
+        tx, tz = fx - ix, fz - iz
+        sx, sz = tx * tx * (3 - 2 * tx), tz * tz * (3 - 2 * tz)
+        a = lattice(ix, iz)
+        b = lattice(ix + 1, iz)
+        c = lattice(ix, iz + 1)
+        d = lattice(ix + 1, iz + 1)
+        h += amp * ((a * (1 - sx) + b * sx) * (1 - sz) + (c * (1 - sx) + d * sx) * sz)
+        amp *= gain
+        freq *= lacunarity
+    return h
+
+
+# ---------------------------------------------------------------- generators
+def cornell_box(seed=1, width=256, height=256, spp=16, max_depth=6):
+    """Cornell-box-like room: 5 quads of a 2x2x2 room, two boxes rotated about
+    y by seeded angles, a 0.5x0.5 ceiling light of radiance (17, 12, 4).
+    Lambertian only; 36 triangles."""
+    rng = np.random.default_rng(seed)
+    s = SceneDesc()
+    s.name = "cornell_box"
+    refl = lambda: tuple(rng.uniform(0.1, 0.8, 3))
+    m_floor, m_ceil, m_back = (s.add_material(capi.MAT_LAMBERTIAN, refl()) for _ in range(3))
+    m_red = s.add_material(capi.MAT_LAMBERTIAN, (0.63, 0.065, 0.05))
+    m_green = s.add_material(capi.MAT_LAMBERTIAN, (0.14, 0.45, 0.091))
+    m_box1, m_box2 = s.add_material(capi.MAT_LAMBERTIAN, refl()), s.add_material(capi.MAT_LAMBERTIAN, refl())
+    m_light = s.add_material(capi.MAT_LAMBERTIAN, (0.0, 0.0, 0.0))
+    # room interior faces (normals point inward)
+    walls = [
+        (quad((-1, 0, -1), (-1, 0, 1), (1, 0, 1), (1, 0, -1)), m_floor),  # floor, +y
+        (quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1)), m_ceil),   # ceiling, -y
+        (quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1)), m_back),  # back, +z
+        (quad((-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (-1, 0, 1)), m_red),   # left, +x
+        (quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1)), m_green),     # right, -x
+    ]
+    for (P, I, N), m in walls:
+        s.add_mesh(P, I, m, normals=N)
+    a1, a2 = rng.uniform(0, 90, 2)
+    P, I, N = box((-0.35, 0.6, -0.3), (0.55, 1.2, 0.55), a1)
+    s.add_mesh(P, I, m_box1, normals=N)
+    P, I, N = box((0.4, 0.3, 0.35), (0.55, 0.6, 0.55), a2)
+    s.add_mesh(P, I, m_box2, normals=N)
+    P, I, N = quad((-0.25, 1.999, -0.25), (0.25, 1.999, -0.25), (0.25, 1.999, 0.25), (-0.25, 1.999, 0.25))
+    lid = s.add_mesh(P, I, m_light, normals=N)
+    s.add_area_light(lid, (17.0, 12.0, 4.0))
+    s.camera.update(cam_to_world=look_at((0, 1, 3.8), (0, 1, 0), (0, 1, 0)), fov=40.0,
+                    width=width, height=height, filter_type=capi.FILTER_BOX, filter_param=1.0)
+    s.options.update(samples_per_pixel=spp, max_depth=max_depth, rr_depth=5)
+    s.meta = dict(seed=seed, box_angles=[float(a1), float(a2)])
+    return s
+
+
+def _random_material(s, rng, kind):
+    if kind == 0:
+        return s.add_material(capi.MAT_LAMBERTIAN, tuple(rng.uniform(0, 1, 3)))
+    if kind == 1:
+        return s.add_material(capi.MAT_ROUGH_PLASTIC, tuple(rng.uniform(0, 1, 3)),
+                              tuple(rng.uniform(0, 1, 3)), roughness=float(rng.uniform(0.05, 0.6)), eta=1.5)
+    p = rng.uniform(0, 1, 12)
+    return s.add_material(capi.MAT_DISNEY_BSDF, tuple(rng.uniform(0, 1, 3)), eta=1.5,
+                          specular_transmission=p[0], metallic=p[1], subsurface=p[2], specular=p[3],
+                          roughness=p[4], specular_tint=p[5], anisotropic=p[6], sheen=p[7], sheen_tint=p[8],
+                          clearcoat=p[9], clearcoat_gloss=p[10])
+
+
+_ICO5 = None
+
+
+def terrain_spheres(seed=7, width=1920, height=1080, spp=64, max_depth=8, grid=(1000, 500),
+                    n_spheres=64, sphere_subdiv=5, n_lights=1024, env=(0.03, 0.04, 0.06)):
+    """Large-mesh scene: a seeded fBm heightfield of grid[0] x grid[1] vertices,
+    n_spheres icospheres (flattened instances), n_lights one-triangle
+    emitters of log-uniform power, and a constant environment."""
+    global _ICO5
+    rng = np.random.default_rng(seed)
+    s = SceneDesc()
+    s.name = "terrain_spheres"
+    nx, nz = grid
+    ext_x, ext_z = 40.0, 20.0
+    xs = np.linspace(-ext_x / 2, ext_x / 2, nx)
+    zs = np.linspace(-ext_z / 2, ext_z / 2, nz)
+    X, Z = np.meshgrid(xs, zs, indexing="xy")  # (nz, nx)
+    H = 1.6 * value_noise_fbm(X, Z, rng)
+    P = np.stack([X, H, Z], -1).reshape(-1, 3)
+    # smooth normals from central differences
+    dhdx = np.gradient(H, xs, axis=1)
+    dhdz = np.gradient(H, zs, axis=0)
+    N = np.stack([-dhdx, np.ones_like(H), -dhdz], -1).reshape(-1, 3)
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    ii, jj = np.meshgrid(np.arange(nz - 1), np.arange(nx - 1), indexing="ij")
+    v00 = (ii * nx + jj).ravel()
+    v01 = v00 + 1
+    v10 = v00 + nx
+    v11 = v10 + 1
+    # (v00, v10, v11) has normal +y for increasing z rows
+    I = np.concatenate([np.stack([v00, v10, v11], 1), np.stack([v00, v11, v01], 1)]).astype(np.int32)
+    m_terrain = _random_material(s, rng, int(rng.integers(0, 2)))
+    s.add_mesh(P, I, m_terrain, normals=N)
+    if _ICO5 is None or _ICO5[0] != sphere_subdiv:
+        _ICO5 = (sphere_subdiv,) + icosphere(sphere_subdiv)
+    _, SP, SI = _ICO5
+
+    def height_at(x, z):
+        j = np.clip(np.searchsorted(xs, x), 0, nx - 1)
+        i = np.clip(np.searchsorted(zs, z), 0, nz - 1)
+        return H[i, j]
+
+    for k in range(n_spheres):
+        r = float(rng.uniform(0.4, 1.3))
+        cx, cz = float(rng.uniform(-17, 17)), float(rng.uniform(-8.5, 8.5))
+        cy = float(height_at(cx, cz)) + r * float(rng.uniform(0.6, 1.4))
+        m = _random_material(s, rng, k % 3)
+        s.add_mesh(SP * r + np.array([cx, cy, cz]), SI, m, normals=SP)
+    m_black = s.add_material(capi.MAT_LAMBERTIAN, (0.0, 0.0, 0.0))
+    for k in range(n_lights):
+        c = np.array([rng.uniform(-19, 19), rng.uniform(5.0, 9.0), rng.uniform(-9.5, 9.5)])
+        size = rng.uniform(0.15, 0.45)
+        a = rng.uniform(0, 2 * np.pi)
+        p0 = c + size * np.array([np.cos(a), 0, np.sin(a)])
+        p1 = c + size * np.array([np.cos(a + 2.1), 0, np.sin(a + 2.1)])
+        p2 = c + size * np.array([np.cos(a + 4.2), 0, np.sin(a + 4.2)])
+        tri = np.array([p0, p1, p2], np.float32)
+        nrm = np.cross(tri[1] - tri[0], tri[2] - tri[0]).astype(np.float64)
+        if nrm[1] > 0:  # face downwards
+            tri = tri[[0, 2, 1]]
+        nrm = np.cross(tri[1] - tri[0], tri[2] - tri[0]).astype(np.float64)
+        area = 0.5 * np.linalg.norm(nrm)
+        nrm /= np.linalg.norm(nrm)
+        power = 10 ** rng.uniform(-1, 2)
+        col = rng.uniform(0.3, 1.0, 3)
+        lum = col @ np.array([0.212671, 0.715160, 0.072169])
+        # light_power = luminance(I) * area * pi (diffuse_area_light.inl:1-3)
+        intensity = col / lum * power / (area * np.pi)
+        sid = s.add_mesh(tri, np.array([[0, 1, 2]], np.int32), m_black,
+                         normals=np.tile(nrm.astype(np.float32), (3, 1)))
+        s.add_area_light(sid, tuple(float(v) for v in intensity))
+    s.add_constant_envmap(env)
+    s.camera.update(cam_to_world=look_at((0, 9, 24), (0, 0, -1), (0, 1, 0)), fov=62.0,
+                    width=width, height=height, filter_type=capi.FILTER_BOX, filter_param=1.0)
+    s.options.update(samples_per_pixel=spp, max_depth=max_depth, rr_depth=5)
+    s.meta = dict(seed=seed, grid=list(grid), n_spheres=n_spheres, sphere_subdiv=sphere_subdiv,
+                  n_lights=n_lights, env=list(env))
+    return s
+
+
+def nested_media(seed=11, g=0.7, width=1024, height=1024, spp=128, max_depth=16, sphere_res=224):
+    """Volumetric scene: a sphere_res^2*2-triangle index-matched sphere holding
+    a homogeneous medium (sigma_a ~ U[0.01,0.5]^3, sigma_s ~ U[0.5,4]^3),
+    nested in a RoughDielectric sphere holding a second medium, over a
+    Lambertian floor, lit by two area lights; Henyey-Greenstein phase."""
+    rng = np.random.default_rng(seed)
+    s = SceneDesc()
+    s.name = "nested_media"
+    med_in = s.add_medium(tuple(rng.uniform(0.01, 0.5, 3)), tuple(rng.uniform(0.5, 4.0, 3)),
+                          capi.PHASE_HENYEY_GREENSTEIN, g)
+    med_out = s.add_medium(tuple(rng.uniform(0.01, 0.1, 3)), tuple(rng.uniform(0.05, 0.3, 3)),
+                           capi.PHASE_HENYEY_GREENSTEIN, g)
+    P, I = uv_sphere(sphere_res, sphere_res)
+    s.add_mesh(P * 0.8 + np.array([0, 1.2, 0]), I, -1, normals=P,
+               interior_medium_id=med_in, exterior_medium_id=med_out)
+    m_glass = s.add_material(capi.MAT_ROUGH_DIELECTRIC, (1, 1, 1), (1, 1, 1), roughness=0.1, eta=1.5)
+    SP, SI = icosphere(4)
+    s.add_mesh(SP * 1.15 + np.array([0, 1.2, 0]), SI, m_glass, normals=SP,
+               interior_medium_id=med_out, exterior_medium_id=-1)
+    m_floor = s.add_material(capi.MAT_LAMBERTIAN, tuple(rng.uniform(0.3, 0.8, 3)))
+    Pf, If, Nf = quad((-6, 0, -6), (-6, 0, 6), (6, 0, 6), (6, 0, -6))
+    s.add_mesh(Pf, If, m_floor, normals=Nf)
+    m_black = s.add_material(capi.MAT_LAMBERTIAN, (0, 0, 0))
+    for c, col in (((-2.5, 4.0, 1.5), (18.0, 14.0, 10.0)), ((2.0, 3.0, -2.0), (4.0, 6.0, 12.0))):
+        cx, cy, cz = c
+        Pl, Il, Nl = quad((cx - 0.4, cy, cz - 0.4), (cx + 0.4, cy, cz - 0.4), (cx + 0.4, cy, cz + 0.4),
+                          (cx - 0.4, cy, cz + 0.4))
+        lid = s.add_mesh(Pl, Il, m_black, normals=Nl)
+        s.add_area_light(lid, col)
+    s.camera.update(cam_to_world=look_at((0, 2.2, 5.5), (0, 1.0, 0), (0, 1, 0)), fov=45.0,
+                    width=width, height=height, filter_type=capi.FILTER_BOX, filter_param=1.0)
+    s.options.update(integrator=capi.INTEGRATOR_VOLPATH, samples_per_pixel=spp, max_depth=max_depth,
+                     rr_depth=5)
+    s.meta = dict(seed=seed, g=g, sphere_res=sphere_res)
+    return s

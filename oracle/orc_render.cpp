@@ -1,0 +1,1406 @@
+// ORACLE -- test infrastructure only (see orc_core.h).
+//
+// FP64 restatement of lajolla's scene model, intersection, lights and
+// integrators, with a CPU SAH BVH standing in for Embree (which the reference
+// links as a binary that is absent here) and an exact brute-force
+// intersector for hit-parity checks.
+//
+// Two RNG plans are offered:
+//   plan 0 -- the reference's own: 16x16 tiles, one PCG stream per tile
+//             (stream = tile_y * ntx + tile_x, default seed), consumed
+//             sequentially over the tile's pixels and samples
+//             (src/render.cpp:71-101).
+//   plan 1 -- one PCG stream per camera path: stream = s * W * H + pixel,
+//             seed = options.seed.  Draw order inside a path is the
+//             reference's; the HIP wavefront tracer uses the same plan, so
+//             the two agree sample by sample up to FP32/FP64 rounding.
+#include "orc_core.h"
+#include "../include/lajolla_amd.h"
+
+#include <atomic>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+
+namespace orc {
+
+// ------------------------------------------------------------------ scene
+struct Shape {  // shape.h:20-51
+    int type = 1;
+    int material_id = -1, area_light_id = -1, interior_medium_id = -1, exterior_medium_id = -1;
+    std::vector<V3> positions, normals;
+    std::vector<V2> uvs;
+    std::vector<int> indices;  // 3 per triangle
+    V3 center;
+    Real radius = 0;
+    Real total_area = 0;
+    TableDist1D triangle_sampler;
+};
+struct Light {  // light.h:12-30
+    int type = 0;
+    int shape_id = -1;
+    V3 intensity;
+    V3 env_value;  // 1x1 image texel
+    M4 to_world, to_local;
+    Real scale = 1;
+    TableDist2D dist;
+};
+struct Medium {  // medium.h:10-22 (homogeneous)
+    V3 sigma_a, sigma_s;
+    Phase phase;
+};
+struct PathVertex {  // intersection.h:15-35
+    V3 position, geometric_normal;
+    Frame shading_frame;
+    V2 st, uv;
+    int shape_id = -1, primitive_id = -1, material_id = -1;
+    int interior_medium_id = -1, exterior_medium_id = -1;
+};
+struct Prim {
+    int shape, local;
+};
+struct BNode {
+    Real lo[3], hi[3];
+    int left = -1, right = -1;  // children (inner)
+    int first = 0, count = 0;   // leaf prims
+};
+struct Scene {
+    Camera camera;
+    std::vector<Material> materials;
+    std::vector<Shape> shapes;
+    std::vector<Light> lights;
+    std::vector<Medium> media;
+    int envmap_light_id = -1;
+    Real bound_radius = 0;
+    V3 bound_center;
+    ljx_render_options options{};
+    TableDist1D light_dist;
+    std::vector<Prim> prims;
+    std::vector<BNode> nodes;
+};
+
+inline Real shadow_eps(const Scene &s) { return std::min(s.bound_radius * Real(1e-5), Real(0.01)); }
+
+// ------------------------------------------------------- ray/prim tests
+// The reference hands Embree an FP32 ray (intersection.cpp:15-31).
+inline Ray to_f32(const Ray &r) {
+    Ray q;
+    q.org = V3{Real(float(r.org.x)), Real(float(r.org.y)), Real(float(r.org.z))};
+    q.dir = V3{Real(float(r.dir.x)), Real(float(r.dir.y)), Real(float(r.dir.z))};
+    q.tnear = Real(float(r.tnear));
+    q.tfar = Real(float(r.tfar));
+    return q;
+}
+struct HitRec {
+    Real t = inf();
+    Real u = 0, v = 0;
+    int prim = -1;
+    V3 ng;
+    Real min_bary = 0;
+};
+// Moller-Trumbore in FP64 on the FP32 vertices; u weights v1 and v weights v2.
+inline bool tri_hit(const Shape &sh, int tri, const Ray &r, Real tmin, Real tmax, HitRec &h) {
+    const V3 &p0 = sh.positions[sh.indices[3 * tri]];
+    const V3 &p1 = sh.positions[sh.indices[3 * tri + 1]];
+    const V3 &p2 = sh.positions[sh.indices[3 * tri + 2]];
+    V3 e1 = p1 - p0, e2 = p2 - p0;
+    V3 pv = cross(r.dir, e2);
+    Real det = dot(e1, pv);
+    if (det == 0) return false;
+    Real inv = 1 / det;
+    V3 tv = r.org - p0;
+    Real u = dot(tv, pv) * inv;
+    if (u < 0 || u > 1) return false;
+    V3 qv = cross(tv, e1);
+    Real v = dot(r.dir, qv) * inv;
+    if (v < 0 || u + v > 1) return false;
+    Real t = dot(e2, qv) * inv;
+    if (!(t >= tmin && t <= tmax)) return false;
+    h.t = t;
+    h.u = u;
+    h.v = v;
+    h.ng = cross(e1, e2);
+    h.min_bary = std::min(std::min(u, v), 1 - u - v);
+    return true;
+}
+// sphere.inl:12-38, 40-106 (numerically stable quadratic; st from azimuth/elevation)
+inline bool solve_quadratic(Real a, Real b, Real c, Real *t0, Real *t1) {
+    if (a == 0) {
+        if (b == 0) return false;
+        *t0 = *t1 = -c / b;
+        return true;
+    }
+    Real disc = b * b - 4 * a * c;
+    if (disc < 0) return false;
+    Real rd = std::sqrt(disc);
+    if (b >= 0) {
+        *t0 = (-b - rd) / (2 * a);
+        *t1 = 2 * c / (-b - rd);
+    } else {
+        *t0 = 2 * c / (-b + rd);
+        *t1 = (-b + rd) / (2 * a);
+    }
+    return true;
+}
+inline bool sphere_hit(const Shape &sh, const Ray &r, Real tmin, Real tmax, HitRec &h) {
+    V3 v = r.org - sh.center;
+    Real A = dot(r.dir, r.dir), B = 2 * dot(r.dir, v), C = dot(v, v) - sh.radius * sh.radius;
+    Real t0, t1;
+    if (!solve_quadratic(A, B, C, &t0, &t1)) return false;
+    if (t0 > t1) std::swap(t0, t1);
+    Real t = -1;
+    if (t0 >= tmin && t0 < tmax) t = t0;
+    if (t1 >= tmin && t1 < tmax && t < 0) t = t1;
+    if (!(t >= tmin && t < tmax)) return false;
+    V3 p = r.org + t * r.dir;
+    V3 n = p - sh.center;
+    V3 c = n / sh.radius;
+    Real elevation = std::acos(std::clamp(c.y, Real(-1), Real(1)));
+    Real azimuth = std::atan2(c.z, c.x);
+    h.t = t;
+    h.u = azimuth / kTwoPi;
+    h.v = elevation / kPi;
+    h.ng = n;
+    h.min_bary = 1;
+    return true;
+}
+inline bool prim_hit(const Scene &s, int pi, const Ray &r, Real tmin, Real tmax, HitRec &h) {
+    const Prim &p = s.prims[pi];
+    const Shape &sh = s.shapes[p.shape];
+    bool ok = sh.type == LJX_SHAPE_SPHERE ? sphere_hit(sh, r, tmin, tmax, h) : tri_hit(sh, p.local, r, tmin, tmax, h);
+    if (ok) h.prim = pi;
+    return ok;
+}
+
+// ----------------------------------------------------------------- BVH
+struct BuildPrim {
+    Real lo[3], hi[3], c[3];
+    int id;
+};
+static void prim_bounds(const Scene &s, int pi, BuildPrim &b) {
+    const Prim &p = s.prims[pi];
+    const Shape &sh = s.shapes[p.shape];
+    b.id = pi;
+    if (sh.type == LJX_SHAPE_SPHERE) {
+        for (int k = 0; k < 3; k++) {
+            b.lo[k] = sh.center[k] - sh.radius;
+            b.hi[k] = sh.center[k] + sh.radius;
+        }
+    } else {
+        for (int k = 0; k < 3; k++) {
+            b.lo[k] = inf();
+            b.hi[k] = -inf();
+        }
+        for (int j = 0; j < 3; j++) {
+            const V3 &q = sh.positions[sh.indices[3 * p.local + j]];
+            for (int k = 0; k < 3; k++) {
+                b.lo[k] = std::min(b.lo[k], q[k]);
+                b.hi[k] = std::max(b.hi[k], q[k]);
+            }
+        }
+    }
+    for (int k = 0; k < 3; k++) b.c[k] = (b.lo[k] + b.hi[k]) / 2;
+}
+static Real half_area(const Real lo[3], const Real hi[3]) {
+    Real d0 = hi[0] - lo[0], d1 = hi[1] - lo[1], d2 = hi[2] - lo[2];
+    if (d0 < 0) return 0;
+    return d0 * d1 + d1 * d2 + d2 * d0;
+}
+static int build_rec(Scene &s, std::vector<BuildPrim> &bp, int begin, int end) {
+    int ni = int(s.nodes.size());
+    s.nodes.push_back(BNode{});
+    BNode nd;
+    for (int k = 0; k < 3; k++) {
+        nd.lo[k] = inf();
+        nd.hi[k] = -inf();
+    }
+    Real clo[3] = {inf(), inf(), inf()}, chi[3] = {-inf(), -inf(), -inf()};
+    for (int i = begin; i < end; i++)
+        for (int k = 0; k < 3; k++) {
+            nd.lo[k] = std::min(nd.lo[k], bp[i].lo[k]);
+            nd.hi[k] = std::max(nd.hi[k], bp[i].hi[k]);
+            clo[k] = std::min(clo[k], bp[i].c[k]);
+            chi[k] = std::max(chi[k], bp[i].c[k]);
+        }
+    int n = end - begin;
+    auto make_leaf = [&]() {
+        nd.first = begin;
+        nd.count = n;
+        s.nodes[ni] = nd;
+        return ni;
+    };
+    if (n <= 2) return make_leaf();
+    constexpr int B = 16;
+    Real best = inf();
+    int best_axis = -1, best_split = -1;
+    for (int ax = 0; ax < 3; ax++) {
+        Real ext = chi[ax] - clo[ax];
+        if (ext <= 0) continue;
+        int cnt[B] = {0};
+        Real blo[B][3], bhi[B][3];
+        for (int b = 0; b < B; b++)
+            for (int k = 0; k < 3; k++) {
+                blo[b][k] = inf();
+                bhi[b][k] = -inf();
+            }
+        for (int i = begin; i < end; i++) {
+            int b = std::min(B - 1, int(B * (bp[i].c[ax] - clo[ax]) / ext));
+            cnt[b]++;
+            for (int k = 0; k < 3; k++) {
+                blo[b][k] = std::min(blo[b][k], bp[i].lo[k]);
+                bhi[b][k] = std::max(bhi[b][k], bp[i].hi[k]);
+            }
+        }
+        Real rarea[B];
+        int rcnt[B];
+        Real lo[3] = {inf(), inf(), inf()}, hi[3] = {-inf(), -inf(), -inf()};
+        int c = 0;
+        for (int b = B - 1; b > 0; b--) {
+            c += cnt[b];
+            for (int k = 0; k < 3; k++) {
+                lo[k] = std::min(lo[k], blo[b][k]);
+                hi[k] = std::max(hi[k], bhi[b][k]);
+            }
+            rarea[b] = half_area(lo, hi);
+            rcnt[b] = c;
+        }
+        for (int k = 0; k < 3; k++) {
+            lo[k] = inf();
+            hi[k] = -inf();
+        }
+        c = 0;
+        for (int b = 0; b < B - 1; b++) {
+            c += cnt[b];
+            for (int k = 0; k < 3; k++) {
+                lo[k] = std::min(lo[k], blo[b][k]);
+                hi[k] = std::max(hi[k], bhi[b][k]);
+            }
+            if (c == 0 || rcnt[b + 1] == 0) continue;
+            Real cost = c * half_area(lo, hi) + rcnt[b + 1] * rarea[b + 1];
+            if (cost < best) {
+                best = cost;
+                best_axis = ax;
+                best_split = b;
+            }
+        }
+    }
+    int mid;
+    if (best_axis < 0) {
+        if (n <= 8) return make_leaf();
+        mid = begin + n / 2;  // all centroids coincide: split by count
+    } else {
+        Real leaf_cost = n * half_area(nd.lo, nd.hi);
+        if (n <= 4 && best + half_area(nd.lo, nd.hi) >= leaf_cost) return make_leaf();
+        Real ext = chi[best_axis] - clo[best_axis];
+        auto it = std::partition(bp.begin() + begin, bp.begin() + end, [&](const BuildPrim &p) {
+            int b = std::min(B - 1, int(B * (p.c[best_axis] - clo[best_axis]) / ext));
+            return b <= best_split;
+        });
+        mid = int(it - bp.begin());
+        if (mid == begin || mid == end) mid = begin + n / 2;
+    }
+    int l = build_rec(s, bp, begin, mid);
+    int r = build_rec(s, bp, mid, end);
+    nd.left = l;
+    nd.right = r;
+    s.nodes[ni] = nd;
+    return ni;
+}
+static void build_bvh(Scene &s) {
+    s.prims.clear();
+    for (int i = 0; i < int(s.shapes.size()); i++) {
+        const Shape &sh = s.shapes[i];
+        if (sh.type == LJX_SHAPE_SPHERE) s.prims.push_back({i, 0});
+        else
+            for (int t = 0; t < int(sh.indices.size() / 3); t++) s.prims.push_back({i, t});
+    }
+    std::vector<BuildPrim> bp(s.prims.size());
+    for (size_t i = 0; i < bp.size(); i++) prim_bounds(s, int(i), bp[i]);
+    s.nodes.clear();
+    s.nodes.reserve(bp.size() * 2);
+    if (!bp.empty()) build_rec(s, bp, 0, int(bp.size()));
+    std::vector<Prim> ordered(bp.size());
+    for (size_t i = 0; i < bp.size(); i++) ordered[i] = s.prims[bp[i].id];
+    s.prims.swap(ordered);
+}
+// Slab test with the conservative 1 + 2*gamma(3) widening of Ize (2013).
+inline bool box_hit(const BNode &n, const Ray &r, const V3 &inv, Real tmin, Real tmax) {
+    constexpr Real g = 1 + 2 * 3 * (std::numeric_limits<Real>::epsilon() * 0.5) / (1 - 3 * std::numeric_limits<Real>::epsilon() * 0.5);
+    for (int k = 0; k < 3; k++) {
+        Real t0 = (n.lo[k] - r.org[k]) * inv[k];
+        Real t1 = (n.hi[k] - r.org[k]) * inv[k];
+        if (t0 > t1) std::swap(t0, t1);
+        t1 *= g;
+        if (std::isnan(t0) || std::isnan(t1)) continue;  // ray parallel and on a slab plane
+        tmin = std::max(tmin, t0);
+        tmax = std::min(tmax, t1);
+        if (tmin > tmax) return false;
+    }
+    return true;
+}
+static bool traverse(const Scene &s, const Ray &r, bool any, HitRec &best, uint64_t *node_visits = nullptr) {
+    if (s.nodes.empty()) return false;
+    V3 inv{1 / r.dir.x, 1 / r.dir.y, 1 / r.dir.z};
+    int stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    bool found = false;
+    Real tmax = r.tfar;
+    while (sp > 0) {
+        const BNode &n = s.nodes[stack[--sp]];
+        if (node_visits) (*node_visits)++;
+        if (!box_hit(n, r, inv, r.tnear, tmax)) continue;
+        if (n.left < 0) {
+            for (int i = n.first; i < n.first + n.count; i++) {
+                HitRec h;
+                if (prim_hit(s, i, r, r.tnear, tmax, h)) {
+                    found = true;
+                    if (any) {
+                        best = h;
+                        return true;
+                    }
+                    if (h.t < best.t || best.prim < 0) {
+                        best = h;
+                        tmax = h.t;
+                    }
+                }
+            }
+        } else {
+            stack[sp++] = n.right;
+            stack[sp++] = n.left;
+        }
+    }
+    return found;
+}
+static bool brute(const Scene &s, const Ray &r, bool any, HitRec &best) {
+    bool found = false;
+    Real tmax = r.tfar;
+    for (int i = 0; i < int(s.prims.size()); i++) {
+        HitRec h;
+        if (prim_hit(s, i, r, r.tnear, tmax, h)) {
+            found = true;
+            if (any) {
+                best = h;
+                return true;
+            }
+            best = h;
+            tmax = h.t;
+        }
+    }
+    return found;
+}
+
+// ------------------------------------------------------------- shading info
+// triangle_mesh.inl:77-169 and sphere.inl:243-268
+static void shading_info(const Shape &sh, PathVertex &v) {
+    if (sh.type == LJX_SHAPE_SPHERE) {
+        Real r = sh.radius;
+        V3 dpdu{-r * std::sin(v.st.x) * std::sin(v.st.y), r * std::cos(v.st.x) * std::sin(v.st.y), 0};
+        V3 tangent = normalize(dpdu - v.geometric_normal * dot(v.geometric_normal, dpdu));
+        v.shading_frame = Frame(tangent, normalize(cross(v.geometric_normal, tangent)), v.geometric_normal);
+        v.uv = v.st;
+        return;
+    }
+    int i0 = sh.indices[3 * v.primitive_id], i1 = sh.indices[3 * v.primitive_id + 1],
+        i2 = sh.indices[3 * v.primitive_id + 2];
+    V2 uvs[3];
+    if (!sh.uvs.empty()) {
+        uvs[0] = sh.uvs[i0];
+        uvs[1] = sh.uvs[i1];
+        uvs[2] = sh.uvs[i2];
+    } else {
+        uvs[0] = V2{0, 0};
+        uvs[1] = V2{1, 0};
+        uvs[2] = V2{1, 1};
+    }
+    Real s0 = v.st.x, s1 = v.st.y;
+    v.uv = V2{(1 - s0 - s1) * uvs[0].x + s0 * uvs[1].x + s1 * uvs[2].x,
+              (1 - s0 - s1) * uvs[0].y + s0 * uvs[1].y + s1 * uvs[2].y};
+    V3 p0 = sh.positions[i0], p1 = sh.positions[i1], p2 = sh.positions[i2];
+    V2 duvds{uvs[2].x - uvs[0].x, uvs[2].y - uvs[0].y};
+    V2 duvdt{uvs[2].x - uvs[1].x, uvs[2].y - uvs[1].y};
+    Real det = duvds.x * duvdt.y - duvdt.x * duvds.y;
+    Real dsdu = duvdt.y / det, dtdu = -duvds.y / det;
+    V3 dpdu;
+    if (std::fabs(det) > 1e-8f) {
+        V3 dpds = p2 - p0, dpdt = p2 - p1;
+        dpdu = dpds * dsdu + dpdt * dtdu;
+    } else {
+        dpdu = coordinate_system(v.geometric_normal).first;
+    }
+    V3 n = v.geometric_normal;
+    if (!sh.normals.empty()) {
+        n = normalize((1 - s0 - s1) * sh.normals[i0] + s0 * sh.normals[i1] + s1 * sh.normals[i2]);
+    }
+    V3 tangent = normalize(dpdu - n * dot(n, dpdu));
+    V3 bitangent = normalize(cross(n, tangent));
+    v.shading_frame = Frame(tangent, bitangent, n);
+}
+
+// intersection.cpp:7-65
+static bool intersect(const Scene &s, const Ray &ray, PathVertex &v, uint64_t *count) {
+    if (count) (*count)++;
+    Ray r = to_f32(ray);
+    HitRec h;
+    if (!traverse(s, r, false, h)) return false;
+    const Prim &p = s.prims[h.prim];
+    const Shape &sh = s.shapes[p.shape];
+    v = PathVertex{};
+    v.position = ray.org + ray.dir * Real(float(h.t));
+    v.geometric_normal = normalize(h.ng);
+    v.shape_id = p.shape;
+    v.primitive_id = p.local;
+    v.material_id = sh.material_id;
+    v.interior_medium_id = sh.interior_medium_id;
+    v.exterior_medium_id = sh.exterior_medium_id;
+    v.st = V2{Real(float(h.u)), Real(float(h.v))};
+    shading_info(sh, v);
+    if (dot(v.geometric_normal, v.shading_frame.n) < 0) v.geometric_normal = -v.geometric_normal;
+    return true;
+}
+// intersection.cpp:67-85
+static bool occluded(const Scene &s, const Ray &ray, uint64_t *count) {
+    if (count) (*count)++;
+    HitRec h;
+    return traverse(s, to_f32(ray), true, h);
+}
+
+// ------------------------------------------------------------------ lights
+struct PointAndNormal {
+    V3 position, normal;
+};
+// triangle_mesh.inl:24-58 and sphere.inl:164-238
+static PointAndNormal sample_point_on_shape(const Shape &sh, V3 ref, V2 uv, Real w) {
+    if (sh.type == LJX_SHAPE_SPHERE) {
+        V3 c = sh.center;
+        Real r = sh.radius;
+        if (distance_squared(ref, c) < r * r) {
+            Real z = 1 - 2 * uv.x;
+            Real r_ = std::sqrt(std::fmax(Real(0), 1 - z * z));
+            Real phi = 2 * kPi * uv.y;
+            V3 off(r_ * std::cos(phi), r_ * std::sin(phi), z);
+            return {c + r * off, off};
+        }
+        V3 dir_to_center = normalize(c - ref);
+        Frame f(dir_to_center);
+        Real sin_max_sq = r * r / distance_squared(ref, c);
+        Real cos_max = std::sqrt(std::max(Real(0), 1 - sin_max_sq));
+        Real ce = (1 - uv.x) + uv.x * cos_max;
+        Real se = std::sqrt(std::max(Real(0), 1 - ce * ce));
+        Real az = uv.y * 2 * kPi;
+        Real dc = distance(ref, c);
+        Real ds = dc * ce - std::sqrt(std::max(Real(0), r * r - dc * dc * se * se));
+        Real ca = (dc * dc + r * r - ds * ds) / (2 * dc * r);
+        Real sa = std::sqrt(std::max(Real(0), 1 - ca * ca));
+        V3 n = -to_world(f, V3{sa * std::cos(az), sa * std::sin(az), ca});
+        return {r * n + c, n};
+    }
+    int tri = sample(sh.triangle_sampler, w);
+    int i0 = sh.indices[3 * tri], i1 = sh.indices[3 * tri + 1], i2 = sh.indices[3 * tri + 2];
+    V3 v0 = sh.positions[i0], v1 = sh.positions[i1], v2 = sh.positions[i2];
+    V3 e1 = v1 - v0, e2 = v2 - v0;
+    Real a = std::sqrt(std::clamp(uv.x, Real(0), Real(1)));
+    Real b1 = 1 - a, b2 = a * uv.y;
+    V3 ng = normalize(cross(e1, e2));
+    if (!sh.normals.empty()) {
+        V3 sn = normalize((1 - b1 - b2) * sh.normals[i0] + b1 * sh.normals[i1] + b2 * sh.normals[i2]);
+        if (dot(ng, sn) < 0) ng = -ng;
+    }
+    return {v0 + (e1 * b1) + (e2 * b2), ng};
+}
+static Real pdf_point_on_shape(const Shape &sh, const PointAndNormal &pn, V3 ref) {
+    if (sh.type == LJX_SHAPE_SPHERE) {
+        V3 c = sh.center;
+        Real r = sh.radius;
+        if (distance_squared(ref, c) < r * r) return 1 / (4 * kPi * r * r);
+        Real sin_max_sq = r * r / distance_squared(ref, c);
+        Real cos_max = std::sqrt(std::max(Real(0), 1 - sin_max_sq));
+        Real pdf_sa = 1 / (2 * kPi * (1 - cos_max));
+        V3 dir = normalize(pn.position - ref);
+        return pdf_sa * std::fabs(dot(pn.normal, dir)) / distance_squared(ref, pn.position);
+    }
+    return 1 / sh.total_area;
+}
+static Real surface_area(const Shape &sh) {
+    if (sh.type == LJX_SHAPE_SPHERE) return 4 * kPi * sh.radius * sh.radius;
+    return sh.total_area;
+}
+// envmap.inl:1-98 for a 1x1 image texture.
+static V2 env_uv(const Light &l, V3 world_dir) {
+    V3 ld = xform_vector(l.to_local, world_dir);
+    V2 uv{std::atan2(ld.x, -ld.z) * kInvTwoPi, std::acos(std::clamp(ld.y, Real(-1), Real(1))) * kInvPi};
+    if (uv.x < 0) uv.x += 1;
+    return uv;
+}
+static PointAndNormal sample_point_on_light(const Scene &s, const Light &l, V3 ref, V2 uv, Real w) {
+    if (l.type == LJX_LIGHT_DIFFUSE_AREA) return sample_point_on_shape(s.shapes[l.shape_id], ref, uv, w);
+    V2 e = sample(l.dist, uv);
+    Real az = e.x * (2 * kPi), el = e.y * kPi;
+    V3 ld{std::sin(az) * std::sin(el), std::cos(el), -std::cos(az) * std::sin(el)};
+    V3 wd = xform_vector(l.to_world, ld);
+    return {V3{0, 0, 0}, -wd};
+}
+static Real pdf_point_on_light(const Scene &s, const Light &l, const PointAndNormal &pn, V3 ref) {
+    if (l.type == LJX_LIGHT_DIFFUSE_AREA) return pdf_point_on_shape(s.shapes[l.shape_id], pn, ref);
+    V3 wd = -pn.normal;
+    V3 ld = xform_vector(l.to_local, wd);
+    V2 uv = env_uv(l, wd);
+    Real ce = ld.y;
+    Real se = std::sqrt(std::clamp(1 - ce * ce, Real(0), Real(1)));
+    if (se <= 0) return 0;
+    return pdf(l.dist, uv) / (2 * kPi * kPi * se);
+}
+// diffuse_area_light.inl:15-20; envmap.inl:44-72 (1x1: the texel value)
+static V3 light_emission(const Light &l, V3 view_dir, const PointAndNormal &pn) {
+    if (l.type == LJX_LIGHT_DIFFUSE_AREA) {
+        if (dot(pn.normal, view_dir) <= 0) return {};
+        return l.intensity;
+    }
+    return l.env_value * l.scale;
+}
+static Real light_power(const Scene &s, const Light &l) {
+    if (l.type == LJX_LIGHT_DIFFUSE_AREA) return luminance(l.intensity) * surface_area(s.shapes[l.shape_id]) * kPi;
+    return kPi * s.bound_radius * s.bound_radius * l.dist.total_values / (l.dist.width * l.dist.height);
+}
+static V3 vertex_emission(const Scene &s, const PathVertex &v, V3 view_dir) {
+    const Light &l = s.lights[s.shapes[v.shape_id].area_light_id];
+    return light_emission(l, view_dir, PointAndNormal{v.position, v.geometric_normal});
+}
+
+static ShadePoint shade_point(const PathVertex &v) { return ShadePoint{v.geometric_normal, v.shading_frame}; }
+
+struct Counters {
+    uint64_t ext = 0, shadow = 0;
+};
+
+// --------------------------------------------------------- path_tracing
+// path_tracing.h:7-325
+static V3 path_tracing(const Scene &s, int x, int y, Pcg &rng, Counters &cnt) {
+    int w = s.camera.width, h = s.camera.height;
+    V2 screen{(x + pcg_double(rng)) / w, (y + pcg_double(rng)) / h};
+    Ray ray = sample_primary(s.camera, screen);
+    PathVertex vertex;
+    if (!intersect(s, ray, vertex, &cnt.ext)) {
+        if (s.envmap_light_id >= 0) {
+            const Light &env = s.lights[s.envmap_light_id];
+            return light_emission(env, -ray.dir, PointAndNormal{});
+        }
+        return {};
+    }
+    V3 radiance;
+    V3 throughput{1, 1, 1};
+    Real eta_scale = 1;
+    Real eps = shadow_eps(s);
+    if (s.shapes[vertex.shape_id].area_light_id >= 0) radiance += throughput * vertex_emission(s, vertex, -ray.dir);
+    int max_depth = s.options.max_depth;
+    for (int nv = 3; max_depth == -1 || nv <= max_depth + 1; nv++) {
+        const Material &mat = s.materials[vertex.material_id];
+        ShadePoint sp = shade_point(vertex);
+        V2 light_uv{pcg_double(rng), pcg_double(rng)};
+        Real light_w = pcg_double(rng);
+        Real shape_w = pcg_double(rng);
+        int light_id = sample(s.light_dist, light_w);
+        const Light &light = s.lights[light_id];
+        PointAndNormal pl = sample_point_on_light(s, light, vertex.position, light_uv, shape_w);
+        V3 C1;
+        Real w1 = 0;
+        {
+            Real G = 0;
+            V3 dir_light;
+            if (light.type != LJX_LIGHT_ENVMAP) {
+                dir_light = normalize(pl.position - vertex.position);
+                Ray sr{vertex.position, dir_light, eps, (1 - eps) * distance(pl.position, vertex.position)};
+                if (!occluded(s, sr, &cnt.shadow))
+                    G = std::max(-dot(dir_light, pl.normal), Real(0)) / distance_squared(pl.position, vertex.position);
+            } else {
+                dir_light = -pl.normal;
+                Ray sr{vertex.position, dir_light, eps, inf()};
+                if (!occluded(s, sr, &cnt.shadow)) G = 1;
+            }
+            Real p1 = pmf(s.light_dist, light_id) * pdf_point_on_light(s, light, pl, vertex.position);
+            if (G > 0 && p1 > 0) {
+                V3 dir_view = -ray.dir;
+                V3 f = bsdf_eval(mat, dir_view, dir_light, sp);
+                V3 L = light_emission(light, -dir_light, pl);
+                C1 = G * f * L;
+                Real p2 = bsdf_pdf(mat, dir_view, dir_light, sp);
+                p2 *= G;
+                w1 = (p1 * p1) / (p1 * p1 + p2 * p2);
+                C1 /= p1;
+            }
+        }
+        radiance += throughput * C1 * w1;
+        V3 dir_view = -ray.dir;
+        V2 bsdf_uv{pcg_double(rng), pcg_double(rng)};
+        Real bsdf_w = pcg_double(rng);
+        std::optional<BSDFSample> bs = bsdf_sample(mat, dir_view, sp, bsdf_uv, bsdf_w);
+        if (!bs) break;
+        V3 dir_bsdf = bs->dir_out;
+        if (bs->eta != 0) eta_scale /= (bs->eta * bs->eta);
+        Ray bray{vertex.position, dir_bsdf, eps, inf()};
+        PathVertex bv;
+        bool hit = intersect(s, bray, bv, &cnt.ext);
+        Real G;
+        if (hit) G = std::fabs(dot(dir_bsdf, bv.geometric_normal)) / distance_squared(bv.position, vertex.position);
+        else G = 1;
+        V3 f = bsdf_eval(mat, dir_view, dir_bsdf, sp);
+        Real p2 = bsdf_pdf(mat, dir_view, dir_bsdf, sp);
+        if (p2 <= 0) break;
+        p2 *= G;
+        if (hit && s.shapes[bv.shape_id].area_light_id >= 0) {
+            V3 L = vertex_emission(s, bv, -dir_bsdf);
+            V3 C2 = G * f * L;
+            int lid = s.shapes[bv.shape_id].area_light_id;
+            PointAndNormal lp{bv.position, bv.geometric_normal};
+            Real p1 = pmf(s.light_dist, lid) * pdf_point_on_light(s, s.lights[lid], lp, vertex.position);
+            Real w2 = (p2 * p2) / (p1 * p1 + p2 * p2);
+            C2 /= p2;
+            radiance += throughput * C2 * w2;
+        } else if (!hit && s.envmap_light_id >= 0) {
+            const Light &env = s.lights[s.envmap_light_id];
+            V3 L = light_emission(env, -dir_bsdf, PointAndNormal{});
+            V3 C2 = G * f * L;
+            PointAndNormal lp{V3{0, 0, 0}, -dir_bsdf};
+            Real p1 = pmf(s.light_dist, s.envmap_light_id) * pdf_point_on_light(s, env, lp, vertex.position);
+            Real w2 = (p2 * p2) / (p1 * p1 + p2 * p2);
+            C2 /= p2;
+            radiance += throughput * C2 * w2;
+        }
+        if (!hit) break;
+        Real rr_prob = 1;
+        if (nv - 1 >= s.options.rr_depth) {
+            rr_prob = std::min(max3((1 / eta_scale) * throughput), Real(0.95));
+            if (pcg_double(rng) > rr_prob) break;
+        }
+        ray = bray;
+        vertex = bv;
+        throughput = throughput * (G * f) / (p2 * rr_prob);
+    }
+    return radiance;
+}
+
+// ------------------------------------------------------ vol_path_tracing
+// The reference's vol_path_tracing is a stub returning zero
+// (vol_path_tracing.h:59-64).  This follows the handout's final algorithm:
+// homogenised free flight with one-sample spectral MIS
+// (handouts/homework2.tex:713-758), ratio-tracking NEE through index-matched
+// surfaces (:770-812, :460-507), update_medium (:402-409), the MIS caches
+// dir_pdf / nee_p_cache / multi_trans_pdf / never_scatter (:505-557), and
+// Russian roulette from rr_depth (:331-400).
+static int update_medium(V3 dir, const PathVertex &v, int medium) {
+    if (v.interior_medium_id != v.exterior_medium_id) {
+        if (dot(dir, v.geometric_normal) > 0) return v.exterior_medium_id;
+        return v.interior_medium_id;
+    }
+    return medium;
+}
+static V3 vol_nee(const Scene &s, V3 p, V3 dir_view, int medium_id, int bounces, Pcg &rng,
+                  const Material *mat, const ShadePoint *sp, const Phase *phase, Counters &cnt) {
+    V2 light_uv{pcg_double(rng), pcg_double(rng)};
+    Real light_w = pcg_double(rng);
+    Real shape_w = pcg_double(rng);
+    int light_id = sample(s.light_dist, light_w);
+    const Light &light = s.lights[light_id];
+    PointAndNormal pl = sample_point_on_light(s, light, p, light_uv, shape_w);
+    V3 p_prime = pl.position;
+    V3 T_light{1, 1, 1}, p_trans_nee{1, 1, 1}, p_trans_dir{1, 1, 1};
+    int shadow_medium = medium_id;
+    int shadow_bounces = 0;
+    int max_depth = s.options.max_depth;
+    Real eps = shadow_eps(s);
+    V3 p0 = p;
+    V3 dir_light = normalize(p_prime - p);
+    while (true) {
+        Ray sr{p0, dir_light, eps, (1 - eps) * distance(p0, p_prime)};
+        PathVertex iv;
+        bool hit = intersect(s, sr, iv, &cnt.shadow);
+        Real next_t = distance(p0, p_prime);
+        if (hit) next_t = distance(p0, iv.position);
+        if (shadow_medium >= 0) {
+            const Medium &m = s.media[shadow_medium];
+            V3 maj = m.sigma_a + m.sigma_s;
+            Real max_maj = max3(maj);
+            Real u = pcg_double(rng);
+            int ch = std::clamp(int(u * 3), 0, 2);
+            Real accum_t = 0;
+            int it = 0;
+            while (true) {
+                if (maj[ch] <= 0) break;
+                if (it >= s.options.max_null_collisions) break;
+                Real t = -std::log(1 - pcg_double(rng)) / maj[ch];
+                Real dt = next_t - accum_t;
+                accum_t = std::min(accum_t + t, next_t);
+                if (t < dt) {
+                    V3 sigma_t = m.sigma_a + m.sigma_s;
+                    V3 sigma_n = maj - sigma_t;
+                    V3 e = vexp(-1.0 * maj * t);
+                    T_light *= e * sigma_n / max_maj;
+                    p_trans_nee *= e * maj / max_maj;
+                    V3 real_prob = sigma_t / maj;
+                    p_trans_dir *= e * maj * (1 - real_prob) / max_maj;
+                    if (max3(T_light) <= 0) break;
+                } else {
+                    V3 e = vexp(-1.0 * maj * dt);
+                    T_light *= e;
+                    p_trans_nee *= e;
+                    p_trans_dir *= e;
+                    break;
+                }
+                it++;
+            }
+        }
+        if (!hit) break;
+        if (iv.material_id >= 0) return {};
+        shadow_bounces++;
+        if (max_depth != -1 && bounces + shadow_bounces + 1 >= max_depth) return {};
+        shadow_medium = update_medium(dir_light, iv, shadow_medium);
+        p0 = p0 + next_t * dir_light;
+    }
+    if (max3(T_light) > 0) {
+        Real G = std::max(-dot(dir_light, pl.normal), Real(0)) / distance_squared(p, p_prime);
+        if (G <= 0) return {};
+        V3 f = mat ? bsdf_eval(*mat, dir_view, dir_light, *sp) : phase_eval(*phase, dir_view, dir_light);
+        V3 L = light_emission(light, -dir_light, pl);
+        Real pdf_nee = pmf(s.light_dist, light_id) * pdf_point_on_light(s, light, pl, p) * avg(p_trans_nee);
+        if (pdf_nee <= 0) return {};
+        V3 contrib = T_light * G * f * L / pdf_nee;
+        Real pdf_dir = (mat ? bsdf_pdf(*mat, dir_view, dir_light, *sp) : phase_pdf(*phase, dir_view, dir_light)) *
+                       G * avg(p_trans_dir);
+        Real w = (pdf_nee * pdf_nee) / (pdf_nee * pdf_nee + pdf_dir * pdf_dir);
+        return w * contrib;
+    }
+    return {};
+}
+// nee_stream: plan 1 gives NEE its own PCG stream seeded by one draw of the
+// path stream, so that the wavefront shadow kernel can consume it.
+static V3 vol_path_tracing(const Scene &s, int x, int y, Pcg &rng, bool nee_stream, uint64_t path_stream,
+                           Counters &cnt) {
+    int w = s.camera.width, h = s.camera.height;
+    V2 screen{(x + pcg_double(rng)) / w, (y + pcg_double(rng)) / h};
+    Ray ray = sample_primary(s.camera, screen);
+    int medium = s.camera.medium_id;
+    V3 throughput{1, 1, 1}, radiance;
+    int bounces = 0;
+    Real eta_scale = 1;
+    Real dir_pdf = 0;
+    V3 nee_p_cache = ray.org;
+    V3 multi_trans_pdf{1, 1, 1};
+    bool never_scatter = true;
+    int max_depth = s.options.max_depth;
+    Real eps = shadow_eps(s);
+    while (true) {
+        bool scatter = false;
+        PathVertex v;
+        bool hit = intersect(s, ray, v, &cnt.ext);
+        if (medium >= 0) {
+            const Medium &m = s.media[medium];
+            V3 maj = m.sigma_a + m.sigma_s;
+            Real max_maj = max3(maj);
+            Real u = pcg_double(rng);
+            int ch = std::clamp(int(u * 3), 0, 2);
+            Real t_hit = hit ? distance(ray.org, v.position) : inf();
+            Real accum_t = 0;
+            int it = 0;
+            V3 transmittance{1, 1, 1}, trans_dir_pdf{1, 1, 1};
+            while (true) {
+                if (maj[ch] <= 0) break;
+                if (it >= s.options.max_null_collisions) break;
+                Real t = -std::log(1 - pcg_double(rng)) / maj[ch];
+                Real dt = t_hit - accum_t;
+                accum_t = std::min(accum_t + t, t_hit);
+                if (t < dt) {
+                    V3 sigma_t = m.sigma_a + m.sigma_s;
+                    V3 real_prob = sigma_t / maj;
+                    V3 e = vexp(-1.0 * maj * t);
+                    if (pcg_double(rng) < real_prob[ch]) {
+                        scatter = true;
+                        never_scatter = false;
+                        transmittance *= e / max_maj;
+                        trans_dir_pdf *= e * maj * real_prob / max_maj;
+                        break;
+                    }
+                    V3 sigma_n = maj - sigma_t;
+                    transmittance *= e * sigma_n / max_maj;
+                    trans_dir_pdf *= e * maj * (1 - real_prob) / max_maj;
+                } else {
+                    V3 e = vexp(-1.0 * maj * dt);
+                    transmittance *= e;
+                    trans_dir_pdf *= e;
+                    break;
+                }
+                it++;
+            }
+            if (std::isfinite(accum_t)) ray.org = ray.org + accum_t * ray.dir;
+            multi_trans_pdf *= trans_dir_pdf;
+            throughput *= transmittance / avg(trans_dir_pdf);
+        }
+        if (!scatter && !hit) break;
+        if (!scatter && s.shapes[v.shape_id].area_light_id >= 0) {
+            V3 Le = vertex_emission(s, v, -ray.dir);
+            if (never_scatter) {
+                radiance += throughput * Le;
+            } else {
+                int lid = s.shapes[v.shape_id].area_light_id;
+                PointAndNormal lp{v.position, v.geometric_normal};
+                Real pdf_nee = pmf(s.light_dist, lid) * pdf_point_on_light(s, s.lights[lid], lp, nee_p_cache);
+                Real G = std::fabs(dot(ray.dir, v.geometric_normal)) / distance_squared(nee_p_cache, v.position);
+                Real dp = dir_pdf * avg(multi_trans_pdf) * G;
+                Real wgt = (dp * dp) / (dp * dp + pdf_nee * pdf_nee);
+                radiance += throughput * Le * wgt;
+            }
+        }
+        if (max_depth != -1 && bounces == max_depth - 1) break;
+        if (!scatter && v.material_id == -1) {
+            medium = update_medium(ray.dir, v, medium);
+            ray = Ray{v.position, ray.dir, eps, inf()};
+            bounces++;
+            continue;
+        }
+        V3 dir_view = -ray.dir;
+        Pcg nee_rng;
+        Pcg *nr = &rng;
+        if (nee_stream) {
+            uint32_t k = pcg_next(rng);
+            nee_rng = pcg_init(path_stream ^ 0x5bd1e9955bd1e995ULL, uint64_t(k));
+            nr = &nee_rng;
+        }
+        if (scatter) {
+            const Medium &m = s.media[medium];
+            V3 p = ray.org;
+            V3 C = vol_nee(s, p, dir_view, medium, bounces, *nr, nullptr, nullptr, &m.phase, cnt);
+            radiance += throughput * m.sigma_s * C;
+            V2 r{pcg_double(rng), pcg_double(rng)};
+            std::optional<V3> nd = phase_sample(m.phase, dir_view, r);
+            if (!nd) break;
+            V3 f = phase_eval(m.phase, dir_view, *nd);
+            Real pdf = phase_pdf(m.phase, dir_view, *nd);
+            if (pdf <= 0) break;
+            throughput *= f / pdf * m.sigma_s;
+            dir_pdf = pdf;
+            nee_p_cache = p;
+            multi_trans_pdf = V3{1, 1, 1};
+            ray = Ray{p, *nd, 0, inf()};
+        } else {
+            const Material &mat = s.materials[v.material_id];
+            ShadePoint sp = shade_point(v);
+            V3 C = vol_nee(s, v.position, dir_view, medium, bounces, *nr, &mat, &sp, nullptr, cnt);
+            radiance += throughput * C;
+            V2 r{pcg_double(rng), pcg_double(rng)};
+            Real wr = pcg_double(rng);
+            std::optional<BSDFSample> bs = bsdf_sample(mat, dir_view, sp, r, wr);
+            if (!bs) break;
+            if (bs->eta != 0) eta_scale /= (bs->eta * bs->eta);
+            V3 f = bsdf_eval(mat, dir_view, bs->dir_out, sp);
+            Real pdf = bsdf_pdf(mat, dir_view, bs->dir_out, sp);
+            if (pdf <= 0) break;
+            throughput *= f / pdf;
+            dir_pdf = pdf;
+            nee_p_cache = v.position;
+            multi_trans_pdf = V3{1, 1, 1};
+            never_scatter = false;
+            medium = update_medium(bs->dir_out, v, medium);
+            ray = Ray{v.position, bs->dir_out, eps, inf()};
+        }
+        Real rr_prob = 1;
+        if (bounces >= s.options.rr_depth) {
+            rr_prob = std::min(max3((1 / eta_scale) * throughput), Real(0.95));
+            if (pcg_double(rng) > rr_prob) break;
+            throughput /= rr_prob;
+        }
+        bounces++;
+    }
+    return radiance;
+}
+
+// ------------------------------------------------------------- scene build
+static M4 m4_from(const double *a) {
+    M4 m;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) m(i, j) = a[i * 4 + j];
+    return m;
+}
+static V3 v3(const double *a) { return V3{a[0], a[1], a[2]}; }
+
+static std::string g_err;
+
+static int make_scene(const ljx_scene_desc *d, std::unique_ptr<Scene> &out) {
+    auto s = std::make_unique<Scene>();
+    const ljx_camera_desc &c = d->camera;
+    if (c.width <= 0 || c.height <= 0) {
+        g_err = "camera size must be positive";
+        return LJX_ERR_INVALID_ARGUMENT;
+    }
+    s->camera = make_camera(m4_from(c.cam_to_world), c.fov, c.width, c.height,
+                            Filter{int(c.filter_type), c.filter_param}, c.medium_id);
+    for (int i = 0; i < d->num_materials; i++) {
+        const ljx_material_desc &m = d->materials[i];
+        Material q;
+        q.type = int(m.type);
+        q.a = v3(m.color_a);
+        q.b = v3(m.color_b);
+        q.roughness = m.roughness;
+        q.eta = m.eta;
+        q.specular_transmission = m.specular_transmission;
+        q.metallic = m.metallic;
+        q.subsurface = m.subsurface;
+        q.specular = m.specular;
+        q.specular_tint = m.specular_tint;
+        q.anisotropic = m.anisotropic;
+        q.sheen = m.sheen;
+        q.sheen_tint = m.sheen_tint;
+        q.clearcoat = m.clearcoat;
+        q.clearcoat_gloss = m.clearcoat_gloss;
+        s->materials.push_back(q);
+    }
+    for (int i = 0; i < d->num_media; i++) {
+        const ljx_medium_desc &m = d->media[i];
+        Medium q;
+        q.sigma_a = v3(m.sigma_a);
+        q.sigma_s = v3(m.sigma_s);
+        q.phase = Phase{int(m.phase_type), m.g};
+        s->media.push_back(q);
+    }
+    Real lo[3] = {inf(), inf(), inf()}, hi[3] = {-inf(), -inf(), -inf()};
+    for (int i = 0; i < d->num_shapes; i++) {
+        const ljx_shape_desc &sd = d->shapes[i];
+        Shape sh;
+        sh.type = int(sd.type);
+        sh.material_id = sd.material_id;
+        sh.area_light_id = sd.area_light_id;
+        sh.interior_medium_id = sd.interior_medium_id;
+        sh.exterior_medium_id = sd.exterior_medium_id;
+        if (sh.type == LJX_SHAPE_SPHERE) {
+            sh.center = v3(sd.center);
+            sh.radius = sd.radius;
+            for (int k = 0; k < 3; k++) {
+                lo[k] = std::min(lo[k], Real(float(sh.center[k] - sh.radius)));
+                hi[k] = std::max(hi[k], Real(float(sh.center[k] + sh.radius)));
+            }
+        } else {
+            sh.positions.resize(sd.num_vertices);
+            for (uint64_t v = 0; v < sd.num_vertices; v++) {
+                sh.positions[v] = V3{sd.positions[3 * v], sd.positions[3 * v + 1], sd.positions[3 * v + 2]};
+                for (int k = 0; k < 3; k++) {
+                    lo[k] = std::min(lo[k], sh.positions[v][k]);
+                    hi[k] = std::max(hi[k], sh.positions[v][k]);
+                }
+            }
+            sh.indices.assign(sd.indices, sd.indices + 3 * sd.num_triangles);
+            if (sd.normals) {
+                sh.normals.resize(sd.num_vertices);
+                for (uint64_t v = 0; v < sd.num_vertices; v++)
+                    sh.normals[v] = V3{sd.normals[3 * v], sd.normals[3 * v + 1], sd.normals[3 * v + 2]};
+            }
+            if (sd.uvs) {
+                sh.uvs.resize(sd.num_vertices);
+                for (uint64_t v = 0; v < sd.num_vertices; v++) sh.uvs[v] = V2{sd.uvs[2 * v], sd.uvs[2 * v + 1]};
+            }
+            // triangle_mesh.inl:60-75
+            std::vector<Real> areas(sd.num_triangles);
+            Real total = 0;
+            for (uint64_t t = 0; t < sd.num_triangles; t++) {
+                V3 a = sh.positions[sh.indices[3 * t]], b = sh.positions[sh.indices[3 * t + 1]],
+                   cc = sh.positions[sh.indices[3 * t + 2]];
+                areas[t] = length(cross(b - a, cc - a)) / 2;
+                total += areas[t];
+            }
+            sh.triangle_sampler = make_table_dist_1d(areas);
+            sh.total_area = total;
+        }
+        s->shapes.push_back(std::move(sh));
+    }
+    // scene.cpp:30-34 (bounds of the FP32 geometry)
+    V3 vlo{lo[0], lo[1], lo[2]}, vhi{hi[0], hi[1], hi[2]};
+    s->bound_radius = distance(vhi, vlo) / 2;
+    s->bound_center = (vlo + vhi) / Real(2);
+    for (int i = 0; i < d->num_lights; i++) {
+        const ljx_light_desc &l = d->lights[i];
+        Light q;
+        q.type = int(l.type);
+        q.shape_id = l.shape_id;
+        q.intensity = v3(l.intensity);
+        if (q.type == LJX_LIGHT_ENVMAP) {
+            if (l.env_width != 1 || l.env_height != 1 || !l.env_rgb) {
+                g_err = "only 1x1 environment maps are supported";
+                return LJX_ERR_UNSUPPORTED;
+            }
+            q.env_value = V3{l.env_rgb[0], l.env_rgb[1], l.env_rgb[2]};
+            q.to_world = m4_from(l.to_world);
+            q.to_local = inverse(q.to_world);
+            q.scale = l.scale;
+            // envmap.inl:75-98: luminance * sin(elevation) at texel centres
+            std::vector<Real> f{luminance(q.env_value) * std::sin(kPi * Real(0.5))};
+            q.dist = make_table_dist_2d(f, 1, 1);
+        }
+        s->lights.push_back(q);
+    }
+    s->envmap_light_id = d->envmap_light_id;
+    s->options = d->options;
+    std::vector<Real> power(s->lights.size());
+    for (size_t i = 0; i < s->lights.size(); i++) power[i] = light_power(*s, s->lights[i]);
+    s->light_dist = make_table_dist_1d(power);
+    build_bvh(*s);
+    out = std::move(s);
+    return LJX_OK;
+}
+
+template <class F>
+static void parallel_for(size_t n, int threads, F fn) {
+    if (threads <= 0) threads = int(std::thread::hardware_concurrency());
+    if (threads <= 1 || n < 2) {
+        for (size_t i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++)
+        pool.emplace_back([&]() {
+            for (;;) {
+                size_t i = next.fetch_add(1);
+                if (i >= n) break;
+                fn(i);
+            }
+        });
+    for (auto &t : pool) t.join();
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+int orc_last_error(char *buf, size_t cap) {
+    if (!buf || cap == 0) return LJX_ERR_INVALID_ARGUMENT;
+    std::snprintf(buf, cap, "%s", g_err.c_str());
+    return LJX_OK;
+}
+
+int orc_scene_create(const ljx_scene_desc *d, void **out) {
+    if (!d || !out) return LJX_ERR_INVALID_ARGUMENT;
+    std::unique_ptr<Scene> s;
+    int rc = make_scene(d, s);
+    if (rc != LJX_OK) return rc;
+    *out = s.release();
+    return LJX_OK;
+}
+
+int orc_scene_destroy(void *s) {
+    delete static_cast<Scene *>(s);
+    return LJX_OK;
+}
+
+int orc_scene_bounds(void *sp, double *cr) {
+    Scene *s = static_cast<Scene *>(sp);
+    cr[0] = s->bound_center.x;
+    cr[1] = s->bound_center.y;
+    cr[2] = s->bound_center.z;
+    cr[3] = s->bound_radius;
+    return LJX_OK;
+}
+
+// Closest hit of FP32 rays on the FP32 geometry, computed in FP64.
+// min_bary (may be NULL): smallest barycentric of the hit (edge distance).
+// second_t (may be NULL): t of the nearest hit on any other primitive.
+int orc_trace_closest(void *sp, const ljx_ray *rays, size_t n, ljx_hit *hits, int brute_force,
+                      int threads, double *min_bary, double *second_t) {
+    Scene *s = static_cast<Scene *>(sp);
+    parallel_for((n + 255) / 256, threads, [&](size_t blk) {
+        for (size_t i = blk * 256; i < std::min(n, blk * 256 + 256); i++) {
+            const ljx_ray &q = rays[i];
+            Ray r{V3{q.org[0], q.org[1], q.org[2]}, V3{q.dir[0], q.dir[1], q.dir[2]}, q.tnear, q.tfar};
+            HitRec h;
+            bool ok = brute_force ? brute(*s, r, false, h) : traverse(*s, r, false, h);
+            ljx_hit &o = hits[i];
+            if (!ok) {
+                o = ljx_hit{0, 0, 0, -1, -1, {0, 0, 0}};
+                if (min_bary) min_bary[i] = 0;
+                if (second_t) second_t[i] = inf();
+                continue;
+            }
+            const Prim &p = s->prims[h.prim];
+            o.t = float(h.t);
+            o.u = float(h.u);
+            o.v = float(h.v);
+            o.shape_id = p.shape;
+            o.prim_id = p.local;
+            o.ng[0] = float(h.ng.x);
+            o.ng[1] = float(h.ng.y);
+            o.ng[2] = float(h.ng.z);
+            if (min_bary) min_bary[i] = h.min_bary;
+            if (second_t) {
+                // nearest other primitive: re-run with the winner excluded
+                Real best2 = inf();
+                if (brute_force) {
+                    for (int k = 0; k < int(s->prims.size()); k++) {
+                        if (k == h.prim) continue;
+                        HitRec h2;
+                        if (prim_hit(*s, k, r, r.tnear, r.tfar, h2)) best2 = std::min(best2, h2.t);
+                    }
+                } else {
+                    // traverse with a window around the winning t
+                    Ray r2 = r;
+                    r2.tfar = h.t * (1 + 1e-4) + 1e-12;
+                    V3 inv{1 / r.dir.x, 1 / r.dir.y, 1 / r.dir.z};
+                    int stack[128];
+                    int sp2 = 0;
+                    stack[sp2++] = 0;
+                    while (sp2 > 0) {
+                        const BNode &nd = s->nodes[stack[--sp2]];
+                        if (!box_hit(nd, r2, inv, r2.tnear, r2.tfar)) continue;
+                        if (nd.left < 0) {
+                            for (int k = nd.first; k < nd.first + nd.count; k++) {
+                                if (k == h.prim) continue;
+                                HitRec h2;
+                                if (prim_hit(*s, k, r2, r2.tnear, r2.tfar, h2)) best2 = std::min(best2, h2.t);
+                            }
+                        } else {
+                            stack[sp2++] = nd.right;
+                            stack[sp2++] = nd.left;
+                        }
+                    }
+                }
+                second_t[i] = best2;
+            }
+        }
+    });
+    return LJX_OK;
+}
+
+int orc_trace_any(void *sp, const ljx_ray *rays, size_t n, uint8_t *occ, int brute_force, int threads) {
+    Scene *s = static_cast<Scene *>(sp);
+    parallel_for((n + 255) / 256, threads, [&](size_t blk) {
+        for (size_t i = blk * 256; i < std::min(n, blk * 256 + 256); i++) {
+            const ljx_ray &q = rays[i];
+            Ray r{V3{q.org[0], q.org[1], q.org[2]}, V3{q.dir[0], q.dir[1], q.dir[2]}, q.tnear, q.tfar};
+            HitRec h;
+            occ[i] = (brute_force ? brute(*s, r, true, h) : traverse(*s, r, true, h)) ? 1 : 0;
+        }
+    });
+    return LJX_OK;
+}
+
+// Average BVH node visits per closest-hit ray (for byte accounting).
+int orc_bvh_stats(void *sp, int *num_nodes, int *num_prims) {
+    Scene *s = static_cast<Scene *>(sp);
+    *num_nodes = int(s->nodes.size());
+    *num_prims = int(s->prims.size());
+    return LJX_OK;
+}
+
+int orc_bsdf_batch(void *sp, const ljx_bsdf_query *q, size_t n, ljx_bsdf_result *res) {
+    Scene *s = static_cast<Scene *>(sp);
+    for (size_t i = 0; i < n; i++) {
+        const ljx_bsdf_query &a = q[i];
+        if (a.material_id < 0 || a.material_id >= int(s->materials.size())) {
+            g_err = "material id out of range";
+            return LJX_ERR_INVALID_ARGUMENT;
+        }
+        const Material &m = s->materials[a.material_id];
+        ShadePoint sp2{v3(a.geometric_normal), Frame(v3(a.frame_x), v3(a.frame_y), v3(a.frame_n))};
+        bool tv = a.transport_to_view != 0;
+        V3 f = bsdf_eval(m, v3(a.dir_in), v3(a.dir_out), sp2, tv);
+        Real pdf = bsdf_pdf(m, v3(a.dir_in), v3(a.dir_out), sp2, tv);
+        std::optional<BSDFSample> smp = bsdf_sample(m, v3(a.dir_in), sp2, V2{a.rnd_uv[0], a.rnd_uv[1]}, a.rnd_w, tv);
+        ljx_bsdf_result &r = res[i];
+        r.f[0] = f.x;
+        r.f[1] = f.y;
+        r.f[2] = f.z;
+        r.pdf = pdf;
+        r.sample_valid = smp ? 1 : 0;
+        r.sample_dir[0] = smp ? smp->dir_out.x : 0;
+        r.sample_dir[1] = smp ? smp->dir_out.y : 0;
+        r.sample_dir[2] = smp ? smp->dir_out.z : 0;
+        r.sample_eta = smp ? smp->eta : 0;
+        r.sample_roughness = smp ? smp->roughness : 0;
+    }
+    return LJX_OK;
+}
+
+// Renders the image (or the tile range [tile_begin, tile_end) of the 16x16
+// tile grid; pass tile_end <= 0 for all tiles) with RNG plan `plan`.
+// rgb_out: height*width*3 floats; pixels outside the range are left alone.
+// counters (may be NULL): [0] camera paths, [1] extension rays, [2] shadow rays.
+int orc_render(void *sp, const ljx_render_options *opts, int plan, int threads, int tile_begin,
+               int tile_end, float *rgb_out, uint64_t *counters) {
+    Scene *s = static_cast<Scene *>(sp);
+    if (opts) s->options = *opts;
+    int w = s->camera.width, h = s->camera.height;
+    const int ts = 16;
+    int ntx = (w + ts - 1) / ts, nty = (h + ts - 1) / ts;
+    int ntiles = ntx * nty;
+    if (tile_end <= 0 || tile_end > ntiles) tile_end = ntiles;
+    if (tile_begin < 0) tile_begin = 0;
+    bool vol = s->options.integrator == LJX_INTEGRATOR_VOLPATH;
+    if (s->options.integrator != LJX_INTEGRATOR_PATH && !vol) {
+        g_err = "oracle renders the path and volpath integrators only";
+        return LJX_ERR_UNSUPPORTED;
+    }
+    if (vol && s->envmap_light_id >= 0) {
+        g_err = "volpath assumes no environment map (homework2.tex:180)";
+        return LJX_ERR_UNSUPPORTED;
+    }
+    int spp = s->options.samples_per_pixel;
+    std::atomic<uint64_t> n_paths{0}, n_ext{0}, n_sh{0};
+    parallel_for(size_t(tile_end - tile_begin), threads, [&](size_t k) {
+        int tile = tile_begin + int(k);
+        int tx = tile % ntx, ty = tile / ntx;
+        Counters cnt;
+        Pcg rng = pcg_init(uint64_t(ty) * ntx + tx);
+        int x0 = tx * ts, x1 = std::min(x0 + ts, w), y0 = ty * ts, y1 = std::min(y0 + ts, h);
+        uint64_t paths = 0;
+        for (int y = y0; y < y1; y++)
+            for (int x = x0; x < x1; x++) {
+                V3 rad;
+                for (int smp = 0; smp < spp; smp++) {
+                    V3 L;
+                    uint64_t stream = uint64_t(smp) * uint64_t(w) * uint64_t(h) + uint64_t(y) * w + x;
+                    if (plan == 1) {
+                        Pcg prng = pcg_init(stream, s->options.seed);
+                        L = vol ? vol_path_tracing(*s, x, y, prng, true, stream, cnt) : path_tracing(*s, x, y, prng, cnt);
+                    } else {
+                        L = vol ? vol_path_tracing(*s, x, y, rng, false, 0, cnt) : path_tracing(*s, x, y, rng, cnt);
+                    }
+                    // render.cpp:89-94 keeps every sample; :139-143 drops non-finite ones
+                    if (!vol || isfinite3(L)) rad += L;
+                    paths++;
+                }
+                V3 px = rad / Real(spp);
+                size_t o = (size_t(y) * w + x) * 3;
+                rgb_out[o] = float(px.x);
+                rgb_out[o + 1] = float(px.y);
+                rgb_out[o + 2] = float(px.z);
+            }
+        n_paths += paths;
+        n_ext += cnt.ext;
+        n_sh += cnt.shadow;
+    });
+    if (counters) {
+        counters[0] = n_paths;
+        counters[1] = n_ext;
+        counters[2] = n_sh;
+    }
+    return LJX_OK;
+}
+
+// One camera path per (pixel, sample) pair under plan 1; out: n*3 doubles.
+int orc_trace_samples(void *sp, const ljx_render_options *opts, const uint32_t *pixel, const uint32_t *sample_idx,
+                      size_t n, double *out, int threads) {
+    Scene *s = static_cast<Scene *>(sp);
+    if (opts) s->options = *opts;
+    int w = s->camera.width, h = s->camera.height;
+    bool vol = s->options.integrator == LJX_INTEGRATOR_VOLPATH;
+    parallel_for((n + 63) / 64, threads, [&](size_t blk) {
+        Counters cnt;
+        for (size_t i = blk * 64; i < std::min(n, blk * 64 + 64); i++) {
+            uint64_t stream = uint64_t(sample_idx[i]) * uint64_t(w) * uint64_t(h) + pixel[i];
+            Pcg prng = pcg_init(stream, s->options.seed);
+            int x = int(pixel[i] % uint32_t(w)), y = int(pixel[i] / uint32_t(w));
+            V3 L = vol ? vol_path_tracing(*s, x, y, prng, true, stream, cnt) : path_tracing(*s, x, y, prng, cnt);
+            out[3 * i] = L.x;
+            out[3 * i + 1] = L.y;
+            out[3 * i + 2] = L.z;
+        }
+    });
+    return LJX_OK;
+}
+
+// ---- known-answer helpers over the oracle's restated primitives ----------
+int orc_pcg_u32(uint64_t stream, uint64_t seed, size_t n, uint32_t *out) {
+    Pcg r = pcg_init(stream, seed);
+    for (size_t i = 0; i < n; i++) out[i] = pcg_next(r);
+    return LJX_OK;
+}
+int orc_pcg_f64(uint64_t stream, uint64_t seed, size_t n, double *out) {
+    Pcg r = pcg_init(stream, seed);
+    for (size_t i = 0; i < n; i++) out[i] = pcg_double(r);
+    return LJX_OK;
+}
+int orc_pcg_f32(uint64_t stream, uint64_t seed, size_t n, float *out) {
+    Pcg r = pcg_init(stream, seed);
+    for (size_t i = 0; i < n; i++) out[i] = pcg_float(r);
+    return LJX_OK;
+}
+int orc_table1d(const double *f, size_t nf, const double *u, size_t n, int32_t *idx, double *pmf_out,
+                double *cdf_out) {
+    TableDist1D t = make_table_dist_1d(std::vector<Real>(f, f + nf));
+    for (size_t i = 0; i < n; i++) idx[i] = sample(t, u[i]);
+    for (size_t i = 0; i < nf; i++) pmf_out[i] = t.pmf[i];
+    for (size_t i = 0; i <= nf; i++) cdf_out[i] = t.cdf[i];
+    return LJX_OK;
+}
+int orc_table2d(const double *f, int w, int h, const double *uv, size_t n, double *xy_out, double *pdf_out) {
+    TableDist2D t = make_table_dist_2d(std::vector<Real>(f, f + size_t(w) * h), w, h);
+    for (size_t i = 0; i < n; i++) {
+        V2 r = sample(t, V2{uv[2 * i], uv[2 * i + 1]});
+        xy_out[2 * i] = r.x;
+        xy_out[2 * i + 1] = r.y;
+        pdf_out[i] = pdf(t, r);
+    }
+    return LJX_OK;
+}
+// sample_primary for each screen position; out: n*6 doubles (org, dir).
+int orc_camera_rays(const ljx_camera_desc *c, const double *screen, size_t n, double *out) {
+    Camera cam = make_camera(m4_from(c->cam_to_world), c->fov, c->width, c->height,
+                             Filter{int(c->filter_type), c->filter_param}, c->medium_id);
+    for (size_t i = 0; i < n; i++) {
+        Ray r = sample_primary(cam, V2{screen[2 * i], screen[2 * i + 1]});
+        out[6 * i] = r.org.x;
+        out[6 * i + 1] = r.org.y;
+        out[6 * i + 2] = r.org.z;
+        out[6 * i + 3] = r.dir.x;
+        out[6 * i + 4] = r.dir.y;
+        out[6 * i + 5] = r.dir.z;
+    }
+    return LJX_OK;
+}
+int orc_filter_sample(uint32_t type, double param, const double *rnd, size_t n, double *out) {
+    Filter f{int(type), param};
+    for (size_t i = 0; i < n; i++) {
+        V2 o = filter_sample(f, V2{rnd[2 * i], rnd[2 * i + 1]});
+        out[2 * i] = o.x;
+        out[2 * i + 1] = o.y;
+    }
+    return LJX_OK;
+}
+// Phase function eval/pdf/sample; dirs: n*3 in, n*3 out, rnd: n*2.
+int orc_phase_batch(uint32_t type, double g, const double *din, const double *dout, const double *rnd, size_t n,
+                    double *val, double *pdfv, double *smp) {
+    Phase p{int(type), g};
+    for (size_t i = 0; i < n; i++) {
+        V3 a = v3(din + 3 * i), b = v3(dout + 3 * i);
+        val[i] = phase_eval(p, a, b).x;
+        pdfv[i] = phase_pdf(p, a, b);
+        std::optional<V3> sd = phase_sample(p, a, V2{rnd[2 * i], rnd[2 * i + 1]});
+        smp[3 * i] = sd->x;
+        smp[3 * i + 1] = sd->y;
+        smp[3 * i + 2] = sd->z;
+    }
+    return LJX_OK;
+}
+// Light sampling at reference points: out_pn n*6 (position, normal), pdf n.
+int orc_light_sample(void *sp, int light_id, const double *ref, const double *rnd, size_t n, double *out_pn,
+                     double *pdf_out) {
+    Scene *s = static_cast<Scene *>(sp);
+    const Light &l = s->lights[light_id];
+    for (size_t i = 0; i < n; i++) {
+        V3 r = v3(ref + 3 * i);
+        PointAndNormal pn = sample_point_on_light(*s, l, r, V2{rnd[3 * i], rnd[3 * i + 1]}, rnd[3 * i + 2]);
+        out_pn[6 * i] = pn.position.x;
+        out_pn[6 * i + 1] = pn.position.y;
+        out_pn[6 * i + 2] = pn.position.z;
+        out_pn[6 * i + 3] = pn.normal.x;
+        out_pn[6 * i + 4] = pn.normal.y;
+        out_pn[6 * i + 5] = pn.normal.z;
+        pdf_out[i] = pdf_point_on_light(*s, l, pn, r);
+    }
+    return LJX_OK;
+}
+int orc_light_pmf(void *sp, double *out) {
+    Scene *s = static_cast<Scene *>(sp);
+    for (size_t i = 0; i < s->light_dist.pmf.size(); i++) out[i] = s->light_dist.pmf[i];
+    return LJX_OK;
+}
+
+}  // extern "C"
