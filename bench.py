@@ -1,0 +1,172 @@
+"""Throughput benchmark of the wavefront path tracer on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one full render of the large-mesh workload: a seeded fBm
+heightfield (1000x500 vertices) with 64 icospheres (subdivision 5) and 1024
+emissive triangles (2,308,746 triangles), a constant environment,
+Lambertian / RoughPlastic / DisneyBSDF materials, 1920x1080 at 64 samples per
+pixel, max depth 8.  The scene and BVH are resident in HBM before timing; the
+image stays in HBM.  Prints one JSON line.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--cpu-spp", type=int, default=0, help="spp of the bounded CPU baseline sample (0: auto)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--max-paths", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); every box here has one GPU.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from lajolla_public_amd import _capi as capi
+    from lajolla_public_amd import api, scenes
+
+    t0 = time.time()
+    desc = scenes.terrain_spheres(seed=7, spp=args.spp)
+    print(f"[bench] scene generated in {time.time() - t0:.1f}s: {desc.num_triangles} triangles", file=sys.stderr,
+          flush=True)
+    sc = api.Scene(desc)
+    W, H, spp = desc.camera["width"], desc.camera["height"], desc.options["samples_per_pixel"]
+    lib = sc.lib
+    out = api.DeviceBuffer(W * H * 3 * 4)
+    opts = scenes.render_options(**desc.options)
+
+    def run(ex):
+        st = capi.RenderStats()
+        capi.check(lib.ljx_render_device(sc.h, C.byref(opts), C.byref(ex), C.cast(out.ptr, C.c_void_p),
+                                         C.byref(st)), lib)
+        return api.stats_dict(st)
+
+    # instrumented replay: node visits / primitive tests per ray (not timed)
+    inst = run(api.exec_options(collect_traversal=1, max_paths_in_flight=args.max_paths))
+    print(f"[bench] instrumented: {json.dumps({k: inst[k] for k in ('extension_rays', 'node_visits', 'tri_tests', 'shadow_rays', 'shadow_node_visits', 'shadow_tri_tests', 'bvh_depth', 'build_ms')})}",
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        run(api.exec_options(max_paths_in_flight=args.max_paths))
+    ex = api.exec_options(collect_timing=1, max_paths_in_flight=args.max_paths)
+    stats = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        stats.append(run(ex))
+        print(f"[bench] step {k}: {stats[-1]['total_ms']:.1f} ms", file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1000 * elapsed / args.steps
+    samples = W * H * spp
+    value = samples * args.steps / elapsed / 1e6
+    ext_rays = sum(s["extension_rays"] for s in stats)
+    sh_rays = sum(s["shadow_rays"] for s in stats)
+    grays = (ext_rays + sh_rays) / elapsed / 1e9
+
+    # roofline of the dominant kernel (closest-hit traversal): algorithmic
+    # bytes = 48 B per ray (32 B ray in, 16 B hit out) + 64 B per visited node
+    # (two 32 B child boxes) + 36 B per primitive test (three FP32 vertices)
+    ext_ms = sum(s["extend_ms"] for s in stats)
+    ext_launches = sum(s["extend_launches"] for s in stats)
+    nodes_per_ray = inst["node_visits"] / max(1, inst["extension_rays"])
+    tris_per_ray = inst["tri_tests"] / max(1, inst["extension_rays"])
+    bytes_per_ray = 48 + 64 * nodes_per_ray + 36 * tris_per_ray
+    alg_bytes = ext_rays * bytes_per_ray
+    achieved = alg_bytes / (ext_ms / 1000) / 1e9 if ext_ms > 0 else 0.0
+    kernel_ms = {k: sum(s[k] for s in stats) / args.steps for k in ("extend_ms", "shadow_ms", "shade_ms", "other_ms")}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(desc, args.cpu_spp)
+
+    line = {
+        "metric": "Msamples/s (1080p, depth 8)",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 traversal / f64 shading",
+        "data": "synthetic (seeded scene generator)",
+        "config": {
+            "workload": "terrain_spheres: fBm heightfield 1000x500 + 64 icospheres (subdiv 5) + 1024 emissive "
+                        "triangles = 2,308,746 triangles, constant environment, 1920x1080, 64 spp, max depth 8",
+            "width": W, "height": H, "spp": spp, "max_depth": desc.options["max_depth"],
+            "triangles": desc.num_triangles, "seed": desc.meta["seed"],
+        },
+        "grays_per_s": round(grays, 4),
+        "rays_per_step": {"extension": ext_rays // args.steps, "shadow": sh_rays // args.steps},
+        "kernel_ms_per_step": {k: round(v, 2) for k, v in kernel_ms.items()},
+        "roofline": {
+            "kernel": "k_extend (closest-hit traversal)",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_ray": round(bytes_per_ray, 1),
+            "nodes_per_ray": round(nodes_per_ray, 2),
+            "prims_per_ray": round(tris_per_ray, 2),
+            "launches_per_step": ext_launches // args.steps,
+            "avg_launch_ms": round(ext_ms / max(1, ext_launches), 4),
+        },
+        "cpu_baseline": cpu,
+    }
+    if cpu and cpu.get("value"):
+        line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(desc, cpu_spp):
+    """The oracle (lajolla's path_render / path_tracing restated in FP64 with
+    the reference's tile RNG plan, CPU SAH BVH in place of Embree) on the
+    host cores, over every tile of the same image at reduced spp."""
+    from lajolla_public_amd import scenes
+    from oracle import pyoracle as po
+    threads = min(os.cpu_count() or 1, 16)
+    t0 = time.time()
+    orc = po.OracleScene(desc)
+    build_s = time.time() - t0
+    spp = cpu_spp
+    if spp <= 0:
+        # ~0.35 Msamples/s per 8 threads on this scene: aim for ~15 s
+        spp = max(1, min(desc.options["samples_per_pixel"], int(round(15 * 0.045e6 * threads /
+                                                                          (desc.camera["width"] * desc.camera["height"])))))
+    o = dict(desc.options)
+    o["samples_per_pixel"] = spp
+    t0 = time.time()
+    img, cnt = orc.render(options=scenes.render_options(**o), plan=0, threads=threads)
+    dt = time.time() - t0
+    return {
+        "value": round(cnt["paths"] / dt / 1e6, 4),
+        "unit": "Msamples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"every 16x16 tile of the 1920x1080 image at {spp} spp ({cnt['paths']} camera paths, "
+                  f"{dt:.1f} s; BVH build {build_s:.1f} s not timed); lajolla-algorithm CPU baseline "
+                  f"(project BVH, not Embree)",
+        "grays_per_s": round((cnt["ext_rays"] + cnt["shadow_rays"]) / dt / 1e9, 5),
+    }
+
+
+if __name__ == "__main__":
+    main()

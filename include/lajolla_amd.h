@@ -263,9 +263,14 @@ typedef struct ljx_render_stats {
     uint64_t shadow_launches;
     double shade_ms;               /* HIP-event time in the shading kernels */
     double other_ms;               /* raygen, accumulate, bookkeeping kernels */
-    uint64_t node_visits;          /* filled only by instrumented runs */
-    uint64_t tri_tests;
+    uint64_t node_visits;          /* closest-hit kernel; instrumented runs only */
+    uint64_t tri_tests;            /* primitive tests in the closest-hit kernel */
+    uint64_t shadow_node_visits;   /* any-hit kernel; instrumented runs only */
+    uint64_t shadow_tri_tests;
     uint64_t instrumented_rays;
+    double build_ms;               /* scene upload + BVH build of this scene */
+    int32_t bvh_depth;
+    int32_t pad;
 } ljx_render_stats;
 
 /* Render-call switches that do not change results. */

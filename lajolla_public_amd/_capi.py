@@ -193,7 +193,12 @@ class RenderStats(C.Structure):
         ("other_ms", C.c_double),
         ("node_visits", C.c_uint64),
         ("tri_tests", C.c_uint64),
+        ("shadow_node_visits", C.c_uint64),
+        ("shadow_tri_tests", C.c_uint64),
         ("instrumented_rays", C.c_uint64),
+        ("build_ms", C.c_double),
+        ("bvh_depth", C.c_int32),
+        ("pad", C.c_int32),
     ]
 
 

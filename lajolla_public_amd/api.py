@@ -1,0 +1,130 @@
+"""Python host API mirroring lajolla's render path over the C ABI.
+
+    scene = Scene(desc)                 # Scene::Scene (src/scene.cpp:4-53)
+    img = render(scene)                 # render() (src/render.cpp:155-170) -> Image3 (H, W, 3)
+    hits = scene.trace_closest(rays)    # intersect() on Embree's rtcIntersect1
+    occ = scene.trace_any(rays)         # occluded() on rtcOccluded1
+
+Every call goes through liblajolla_amd.so; there is no CPU fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi as capi
+from .scenes import SceneDesc, render_options
+
+
+class Scene:
+    """Device scene: geometry, SAH BVH, light tables, owned by one handle."""
+
+    def __init__(self, desc: SceneDesc, device: int = 0):
+        self.lib = capi.load()
+        self.desc = desc
+        d, keep = desc.to_c()
+        h = C.c_void_p()
+        capi.check(self.lib.ljx_scene_create(C.byref(d), device, C.byref(h)), self.lib)
+        self.h = h
+        self.width = desc.camera["width"]
+        self.height = desc.camera["height"]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ljx_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bounds(self):
+        out = (C.c_double * 4)()
+        capi.check(self.lib.ljx_scene_bounds(self.h, out), self.lib)
+        return np.array(out[:])
+
+    def render(self, options=None, exec_options=None, stats=False, **kw):
+        """Returns the (H, W, 3) float32 image (and a stats dict if asked)."""
+        opts = options
+        if opts is None and kw:
+            o = dict(self.desc.options)
+            o.update(kw)
+            opts = render_options(**o)
+        img = np.zeros((self.height, self.width, 3), np.float32)
+        st = capi.RenderStats()
+        ex = exec_options
+        capi.check(self.lib.ljx_render(self.h, C.byref(opts) if opts is not None else None,
+                                       C.byref(ex) if ex is not None else None,
+                                       img.ctypes.data_as(C.POINTER(C.c_float)), C.byref(st)), self.lib)
+        if stats:
+            return img, stats_dict(st)
+        return img
+
+    def trace_closest(self, rays):
+        n = len(rays)
+        hits = (capi.Hit * n)()
+        capi.check(self.lib.ljx_trace_closest(self.h, rays, n, hits), self.lib)
+        return hits
+
+    def trace_any(self, rays):
+        n = len(rays)
+        occ = np.zeros(n, np.uint8)
+        capi.check(self.lib.ljx_trace_any(self.h, rays, n, occ.ctypes.data_as(C.POINTER(C.c_uint8))), self.lib)
+        return occ
+
+    def bsdf(self, queries):
+        n = len(queries)
+        res = (capi.BSDFResult * n)()
+        capi.check(self.lib.ljx_bsdf_batch(self.h, queries, n, res), self.lib)
+        return res
+
+
+def stats_dict(st):
+    return {name: getattr(st, name) for name, _ in capi.RenderStats._fields_}
+
+
+def exec_options(collect_timing=0, collect_traversal=0, max_paths_in_flight=0, block_size=0, sort_threshold=0):
+    e = capi.ExecOptions()
+    e.collect_timing = collect_timing
+    e.collect_traversal = collect_traversal
+    e.max_paths_in_flight = max_paths_in_flight
+    e.block_size = block_size
+    e.sort_threshold = sort_threshold
+    return e
+
+
+def render(scene: Scene, **kw):
+    """render(scene) -> Image3 as an (H, W, 3) float32 array (src/render.h:9)."""
+    return scene.render(**kw)
+
+
+class DeviceBuffer:
+    """Raw device allocation through the C ABI (no torch dependency)."""
+
+    def __init__(self, nbytes, device=0):
+        self.lib = capi.load()
+        p = C.c_void_p()
+        capi.check(self.lib.ljx_device_alloc(device, nbytes, C.byref(p)), self.lib)
+        self.ptr = p
+        self.nbytes = nbytes
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        capi.check(self.lib.ljx_copy_to_device(self.ptr, arr.ctypes.data_as(C.c_void_p), arr.nbytes), self.lib)
+
+    def download(self, arr):
+        capi.check(self.lib.ljx_copy_to_host(arr.ctypes.data_as(C.c_void_p), self.ptr, arr.nbytes), self.lib)
+        return arr
+
+    def free(self):
+        if self.ptr:
+            self.lib.ljx_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,26 @@
+// Binned-SAH BVH builder (host, multi-threaded) producing the two-box-per-node
+// layout the HIP traversal reads.  Replaces the BVH build inside Embree's
+// rtcCommitScene (src/scene.cpp:20-27).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "ljx_device.h"
+
+namespace ljx {
+
+struct PrimBox {
+    float lo[3], hi[3];
+};
+
+struct BvhBuildResult {
+    std::vector<BvhNode> nodes;   // node 0 is the root (always an inner node)
+    std::vector<int> slot_prim;   // leaf slot -> primitive index
+    int max_depth = 0;
+};
+
+// Builds over n primitive boxes.  max_leaf <= 8.  threads <= 0: all cores.
+BvhBuildResult build_bvh_sah(const std::vector<PrimBox> &boxes, int max_leaf, int threads);
+
+}  // namespace ljx

@@ -1,0 +1,750 @@
+// Device-side shading substrate of the MI355X path-tracing core.
+//
+// Shading runs in FP64 like lajolla's Real (src/lajolla.h:23): CDNA4 issues
+// FP64 vector FMAs at half the FP32 rate, and the BSDFs have ill-conditioned
+// terms (GTR2 at alpha^2 = 1e-8, Fresnel near total internal reflection)
+// that FP32 cannot evaluate to 1e-5.  Traversal stays FP32 (ljx_trace.h),
+// as Embree's does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define LJX_HD __host__ __device__ __forceinline__
+
+namespace ljx {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kInvPi = 1.0 / kPi;
+constexpr double kTwoPi = 2.0 * kPi;
+constexpr double kInvTwoPi = 1.0 / kTwoPi;
+constexpr double kInvFourPi = 1.0 / (4.0 * kPi);
+
+// ------------------------------------------------------------------ vectors
+struct D3 {
+    double x, y, z;
+};
+LJX_HD D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+LJX_HD D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+LJX_HD D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+LJX_HD D3 operator-(D3 a) { return {-a.x, -a.y, -a.z}; }
+LJX_HD D3 operator*(D3 a, D3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+LJX_HD D3 operator*(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+LJX_HD D3 operator*(double s, D3 a) { return {a.x * s, a.y * s, a.z * s}; }
+LJX_HD D3 operator/(D3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+LJX_HD D3 operator/(D3 a, D3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+LJX_HD D3 operator+(D3 a, double s) { return {a.x + s, a.y + s, a.z + s}; }
+LJX_HD D3 operator+(double s, D3 a) { return {a.x + s, a.y + s, a.z + s}; }
+LJX_HD D3 operator-(double s, D3 a) { return {s - a.x, s - a.y, s - a.z}; }
+LJX_HD double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+LJX_HD D3 cross(D3 a, D3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+LJX_HD double length(D3 a) { return sqrt(dot(a, a)); }
+LJX_HD double dist2(D3 a, D3 b) { return dot(a - b, a - b); }
+// Zero-length vectors normalise to zero (src/vector.h:249-257).
+LJX_HD D3 normalize(D3 a) {
+    double l = length(a);
+    return l <= 0 ? D3{0, 0, 0} : a / l;
+}
+LJX_HD double maxc(D3 a) { return fmax(fmax(a.x, a.y), a.z); }
+LJX_HD double avgc(D3 a) { return (a.x + a.y + a.z) / 3; }
+LJX_HD double comp(D3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+LJX_HD D3 vexp(D3 a) { return {exp(a.x), exp(a.y), exp(a.z)}; }
+LJX_HD D3 vsqrt0(D3 a) { return {sqrt(fmax(a.x, 0.0)), sqrt(fmax(a.y, 0.0)), sqrt(fmax(a.z, 0.0))}; }
+LJX_HD double luminance(D3 s) { return s.x * 0.212671 + s.y * 0.715160 + s.z * 0.072169; }
+LJX_HD double clampd(double v, double lo, double hi) { return fmin(fmax(v, lo), hi); }
+LJX_HD double pow5(double x) {
+    double x2 = x * x;
+    return x2 * x2 * x;
+}
+
+// ------------------------------------------------------------------- frames
+struct Frame {
+    D3 x, y, n;
+};
+// Frisvad basis with the n.z < -1 + 1e-6 special case (src/frame.h:11-21).
+LJX_HD Frame frame_from_n(D3 n) {
+    Frame f;
+    f.n = n;
+    if (n.z < -1.0 + 1e-6) {
+        f.x = {0, -1, 0};
+        f.y = {-1, 0, 0};
+    } else {
+        double a = 1 / (1 + n.z);
+        double b = -n.x * n.y * a;
+        f.x = {1 - n.x * n.x * a, b, -n.x};
+        f.y = {b, 1 - n.y * n.y * a, -n.y};
+    }
+    return f;
+}
+LJX_HD Frame flip(const Frame &f) { return Frame{-f.x, -f.y, -f.n}; }
+LJX_HD D3 to_local(const Frame &f, D3 v) { return {dot(v, f.x), dot(v, f.y), dot(v, f.n)}; }
+LJX_HD D3 to_world(const Frame &f, D3 v) { return f.x * v.x + f.y * v.y + f.n * v.z; }
+
+// ---------------------------------------------------------------------- PCG
+// XSH-RR with lajolla's constants (src/pcg.h:16-68).  The increment is
+// derived from the stream id; only the 64-bit state travels with a path.
+LJX_HD uint64_t pcg_inc(uint64_t stream) { return (stream << 1u) | 1u; }
+LJX_HD uint32_t pcg_next(uint64_t &state, uint64_t inc) {
+    uint64_t old = state;
+    state = old * 6364136223846793005ULL + (inc | 1);
+    uint32_t xs = uint32_t(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = uint32_t(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31));
+}
+LJX_HD uint64_t pcg_init_state(uint64_t stream, uint64_t seed) {
+    uint64_t inc = pcg_inc(stream);
+    uint64_t st = 0;
+    pcg_next(st, inc);
+    st += seed;
+    pcg_next(st, inc);
+    return st;
+}
+LJX_HD double pcg_double(uint64_t &state, uint64_t inc) {
+    uint64_t bits = (uint64_t(pcg_next(state, inc)) << 20) | 0x3ff0000000000000ULL;
+    return __builtin_bit_cast(double, bits) - 1.0;
+}
+struct Rng {
+    uint64_t state, inc;
+    LJX_HD double next() { return pcg_double(state, inc); }
+    LJX_HD uint32_t next_u32() { return pcg_next(state, inc); }
+};
+
+// -------------------------------------------------------------- device scene
+// Material record: constant-texture parameters in FP64 (src/material.h:12-98).
+struct MatRec {
+    int type;
+    int pad;
+    D3 a, b;
+    double roughness, eta;
+    double specular_transmission, metallic, subsurface, specular, specular_tint, anisotropic, sheen,
+        sheen_tint, clearcoat, clearcoat_gloss;
+};
+// Shape record (src/shape.h:20-51).
+struct ShapeRec {
+    int type;  // 0 sphere, 1 triangle mesh
+    int material_id;
+    int area_light_id;
+    int interior_medium_id;
+    int exterior_medium_id;
+    int has_normals;
+    int has_uvs;
+    int tri_offset;   // first global triangle id
+    int num_tris;
+    int cdf_offset;   // triangle-area CDF (num_tris + 1 doubles), -1 if not a light
+    double center[3];
+    double radius;
+    double total_area;
+};
+// Light record (src/light.h:12-30); envmaps are 1x1 images.
+struct LightRec {
+    int type;  // 0 area, 1 envmap
+    int shape_id;
+    D3 intensity;
+    D3 env_value;
+    double to_world[9], to_local[9];  // 3x3 (vectors only)
+    double scale;
+};
+struct MediumRec {  // src/medium.h:10-22, homogeneous
+    D3 sigma_a, sigma_s;
+    int phase_type;
+    double g;
+};
+struct CameraRec {  // src/camera.h:7-22
+    double sample_to_cam[16];
+    double cam_to_world[16];
+    int width, height;
+    int filter_type;
+    double filter_param;
+    int medium_id;
+};
+
+// BVH node: the two children's boxes (Aila & Laine layout), 64 bytes.
+//   lxy = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
+//   rxy = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
+//   zz  = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
+//   ch  = (child0, child1, -, -); child >= 0: inner node; child < 0: leaf
+//         ~child = (first_slot << 3) | (count - 1)
+struct alignas(16) BvhNode {
+    float4 lxy, rxy, zz;
+    int4 ch;
+};
+
+struct DeviceScene {
+    // traversal
+    const BvhNode *nodes;
+    const float4 *prims;      // 3 float4 per leaf slot: v0|gid, v1|kind, v2|-
+    int num_nodes;
+    int num_slots;
+    // geometry for shading / light sampling
+    const float *vpos;        // xyz per global vertex
+    const float *vnrm;        // xyz per global vertex (0 where the shape has none)
+    const float *vuv;         // uv per global vertex
+    const int *tri_vidx;      // 3 global vertex ids per global triangle
+    const int *prim_shape;    // shape id per global primitive id
+    const int *prim_local;    // primitive id inside its shape
+    const ShapeRec *shapes;
+    const MatRec *materials;
+    const LightRec *lights;
+    const MediumRec *media;
+    const double *light_pmf;  // num_lights
+    const double *light_cdf;  // num_lights + 1
+    const double *tri_cdf;    // concatenated per-light-shape triangle CDFs
+    int num_lights;
+    int envmap_light_id;
+    int num_triangles;
+    int pad0;
+    CameraRec camera;
+    double bound_radius;
+    double eps;  // min(1e-5 * r, 0.01) (src/scene.h:99-105)
+};
+
+// ---------------------------------------------------------------- sampling
+// TableDist1D sampling: upper_bound over the CDF, clamped (src/table_dist.cpp:27-33).
+LJX_HD int sample_cdf(const double *cdf, int n, double u) {
+    int lo = 0, hi = n + 1;  // search [0, n+1) for the first cdf > u
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid + 1;
+        else hi = mid;
+    }
+    int k = lo - 1;
+    return k < 0 ? 0 : (k > n - 1 ? n - 1 : k);
+}
+
+// Cosine hemisphere (src/material.cpp:4-11).
+LJX_HD D3 sample_cos_hemisphere(double u0, double u1) {
+    double phi = kTwoPi * u0;
+    double tmp = sqrt(clampd(1 - u1, 0, 1));
+    double s, c;
+    sincos(phi, &s, &c);
+    return {c * tmp, s * tmp, sqrt(clampd(u1, 0, 1))};
+}
+
+// ------------------------------------------------------------- microfacets
+// src/microfacet.h:23-114
+LJX_HD double fresnel_dielectric(double n_dot_i, double n_dot_t, double eta) {
+    double rs = (n_dot_i - eta * n_dot_t) / (n_dot_i + eta * n_dot_t);
+    double rp = (eta * n_dot_i - n_dot_t) / (eta * n_dot_i + n_dot_t);
+    return (rs * rs + rp * rp) / 2;
+}
+LJX_HD double fresnel_dielectric(double n_dot_i, double eta) {
+    double n_dot_t_sq = 1 - (1 - n_dot_i * n_dot_i) / (eta * eta);
+    if (n_dot_t_sq < 0) return 1;
+    return fresnel_dielectric(fabs(n_dot_i), sqrt(n_dot_t_sq), eta);
+}
+LJX_HD double gtr2(double n_dot_h, double roughness) {
+    double alpha = roughness * roughness;
+    double a2 = alpha * alpha;
+    double t = 1 + (a2 - 1) * n_dot_h * n_dot_h;
+    return a2 / (kPi * t * t);
+}
+LJX_HD double smith_gtr2(D3 v, double roughness) {
+    double alpha = roughness * roughness;
+    double a2 = alpha * alpha;
+    double Lambda = (-1 + sqrt(1 + (v.x * v.x * a2 + v.y * v.y * a2) / (v.z * v.z))) / 2;
+    return 1 / (1 + Lambda);
+}
+// Anisotropic GGX D and Smith G1 (handouts/homework1.tex:175-259).
+LJX_HD double ggx_aniso_D(D3 hl, double ax, double ay) {
+    double t = hl.x * hl.x / (ax * ax) + hl.y * hl.y / (ay * ay) + hl.z * hl.z;
+    return 1 / (kPi * ax * ay * t * t);
+}
+LJX_HD double ggx_aniso_G1(D3 wl, double ax, double ay) {
+    double a = (wl.x * ax) * (wl.x * ax) + (wl.y * ay) * (wl.y * ay);
+    double Lambda = (sqrt(1 + a / (wl.z * wl.z)) - 1) / 2;
+    return 1 / (1 + Lambda);
+}
+// Visible-normal sampling (Heitz 2018), src/microfacet.h:85-114, with
+// separate x/y roughness for the anisotropic Disney lobes.
+LJX_HD D3 sample_vndf(D3 in, double ax, double ay, double u0, double u1) {
+    double sgn = 1;
+    if (in.z < 0) {
+        in = -in;
+        sgn = -1;
+    }
+    D3 hemi = normalize(D3{ax * in.x, ay * in.y, in.z});
+    double r = sqrt(u0);
+    double s, c;
+    sincos(2 * kPi * u1, &s, &c);
+    double t1 = r * c, t2 = r * s;
+    double sh = (1 + hemi.z) / 2;
+    t2 = (1 - sh) * sqrt(1 - t1 * t1) + sh * t2;
+    D3 disk{t1, t2, sqrt(fmax(0.0, 1 - t1 * t1 - t2 * t2))};
+    Frame hf = frame_from_n(hemi);
+    D3 hn = to_world(hf, disk);
+    return sgn * normalize(D3{ax * hn.x, ay * hn.y, fmax(0.0, hn.z)});
+}
+LJX_HD D3 reflect_about(D3 in, D3 h) { return normalize(-in + 2 * dot(in, h) * h); }
+
+// -------------------------------------------------------------- BSDF kit
+struct ShadeGeom {
+    D3 ng;      // geometric normal, flipped to the shading side
+    Frame sf;   // shading frame
+};
+struct BsdfSample {
+    D3 dir;
+    double eta;
+    double roughness;
+    int valid;
+};
+LJX_HD Frame front_frame(const ShadeGeom &g, D3 in) {
+    return dot(g.sf.n, in) < 0 ? flip(g.sf) : g.sf;
+}
+LJX_HD Frame dielectric_frame(const ShadeGeom &g, D3 in) {
+    return dot(g.sf.n, in) * dot(g.ng, in) < 0 ? flip(g.sf) : g.sf;
+}
+LJX_HD double clamp_rough(double r) { return clampd(r, 0.01, 1.0); }
+
+// Lambertian (src/materials/lambertian.inl:1-50); eval includes the cosine.
+LJX_HD D3 lambert_eval(D3 refl, D3 in, D3 out, const ShadeGeom &g) {
+    if (dot(g.ng, in) < 0 || dot(g.ng, out) < 0) return {0, 0, 0};
+    Frame f = front_frame(g, in);
+    return fmax(dot(f.n, out), 0.0) * refl / kPi;
+}
+LJX_HD double cos_pdf(D3 in, D3 out, const ShadeGeom &g) {
+    if (dot(g.ng, in) < 0 || dot(g.ng, out) < 0) return 0;
+    Frame f = front_frame(g, in);
+    return fmax(dot(f.n, out), 0.0) / kPi;
+}
+LJX_HD BsdfSample cos_sample(D3 in, const ShadeGeom &g, double u0, double u1, double rough) {
+    if (dot(g.ng, in) < 0) return BsdfSample{{0, 0, 0}, 0, 0, 0};
+    Frame f = front_frame(g, in);
+    return BsdfSample{to_world(f, sample_cos_hemisphere(u0, u1)), 0, rough, 1};
+}
+
+// RoughPlastic (src/materials/roughplastic.inl:3-161).
+LJX_HD D3 plastic_eval(const MatRec &m, D3 in, D3 out, const ShadeGeom &g) {
+    if (dot(g.ng, in) < 0 || dot(g.ng, out) < 0) return {0, 0, 0};
+    Frame f = front_frame(g, in);
+    D3 h = normalize(in + out);
+    double ndh = dot(f.n, h), ndi = dot(f.n, in), ndo = dot(f.n, out);
+    if (ndo <= 0 || ndh <= 0) return {0, 0, 0};
+    double r = clamp_rough(m.roughness);
+    double Fo = fresnel_dielectric(dot(h, out), m.eta);
+    double D = gtr2(ndh, r);
+    double G = smith_gtr2(to_local(f, in), r) * smith_gtr2(to_local(f, out), r);
+    D3 spec = m.b * (G * Fo * D) / (4 * ndi * ndo);
+    double Fi = fresnel_dielectric(dot(h, in), m.eta);
+    D3 diff = m.a * ((1 - Fo) * (1 - Fi) / kPi);
+    return (spec + diff) * ndo;
+}
+LJX_HD double plastic_pdf(const MatRec &m, D3 in, D3 out, const ShadeGeom &g) {
+    if (dot(g.ng, in) < 0 || dot(g.ng, out) < 0) return 0;
+    Frame f = front_frame(g, in);
+    D3 h = normalize(in + out);
+    double ndi = dot(f.n, in), ndo = dot(f.n, out), ndh = dot(f.n, h);
+    if (ndo <= 0 || ndh <= 0) return 0;
+    double lS = luminance(m.b), lR = luminance(m.a);
+    if (lS + lR <= 0) return 0;
+    double r = clamp_rough(m.roughness);
+    double sp = lS / (lS + lR);
+    double dp = 1 - sp;
+    sp *= smith_gtr2(to_local(f, in), r) * gtr2(ndh, r) / (4 * ndi);
+    dp *= ndo / kPi;
+    return sp + dp;
+}
+LJX_HD BsdfSample plastic_sample(const MatRec &m, D3 in, const ShadeGeom &g, double u0, double u1, double w) {
+    if (dot(g.ng, in) < 0) return BsdfSample{{0, 0, 0}, 0, 0, 0};
+    Frame f = front_frame(g, in);
+    double lS = luminance(m.b), lR = luminance(m.a);
+    if (lS + lR <= 0) return BsdfSample{{0, 0, 0}, 0, 0, 0};
+    if (w < lS / (lS + lR)) {
+        double r = clamp_rough(m.roughness);
+        double alpha = r * r;
+        D3 h = to_world(f, sample_vndf(to_local(f, in), alpha, alpha, u0, u1));
+        return BsdfSample{reflect_about(in, h), 0, r, 1};
+    }
+    return BsdfSample{to_world(f, sample_cos_hemisphere(u0, u1)), 0, 1, 1};
+}
+
+// Microfacet dielectric lobe shared by RoughDielectric
+// (src/materials/roughdielectric.inl:3-177) and the Disney glass lobe
+// (handouts/homework1.tex:336-404).
+struct Glass {
+    D3 Kr, Kt;
+    double eta;
+    int aniso;
+    double rough;   // clamped (isotropic) or raw Disney roughness
+    double ax, ay;
+};
+LJX_HD double glass_D(const Glass &gl, const Frame &f, D3 h) {
+    return gl.aniso ? ggx_aniso_D(to_local(f, h), gl.ax, gl.ay) : gtr2(dot(f.n, h), gl.rough);
+}
+LJX_HD double glass_G1(const Glass &gl, const Frame &f, D3 w) {
+    return gl.aniso ? ggx_aniso_G1(to_local(f, w), gl.ax, gl.ay) : smith_gtr2(to_local(f, w), gl.rough);
+}
+LJX_HD D3 glass_eval(const Glass &gl, D3 in, D3 out, const ShadeGeom &g, bool to_view) {
+    bool refl = dot(g.ng, in) * dot(g.ng, out) > 0;
+    Frame f = dielectric_frame(g, in);
+    double eta = dot(g.ng, in) > 0 ? gl.eta : 1 / gl.eta;
+    D3 h = refl ? normalize(in + out) : normalize(in + out * eta);
+    if (dot(h, f.n) < 0) h = -h;
+    double hdi = dot(h, in);
+    double F = fresnel_dielectric(hdi, eta);
+    double D = glass_D(gl, f, h);
+    double G = glass_G1(gl, f, in) * glass_G1(gl, f, out);
+    if (refl) return gl.Kr * (F * D * G / (4 * fabs(dot(f.n, in))));
+    double eta_factor = to_view ? 1.0 : 1 / (eta * eta);
+    double hdo = dot(h, out);
+    double sd = hdi + eta * hdo;
+    return gl.Kt * (eta_factor * (1 - F) * D * G * eta * eta * fabs(hdo * hdi) / (fabs(dot(f.n, in)) * sd * sd));
+}
+LJX_HD double glass_pdf(const Glass &gl, D3 in, D3 out, const ShadeGeom &g) {
+    bool refl = dot(g.ng, in) * dot(g.ng, out) > 0;
+    Frame f = dielectric_frame(g, in);
+    double eta = dot(g.ng, in) > 0 ? gl.eta : 1 / gl.eta;
+    D3 h = refl ? normalize(in + out) : normalize(in + out * eta);
+    if (dot(h, f.n) < 0) h = -h;
+    double hdi = dot(h, in);
+    double F = fresnel_dielectric(hdi, eta);
+    double D = glass_D(gl, f, h);
+    double Gi = glass_G1(gl, f, in);
+    if (refl) return F * D * Gi / (4 * fabs(dot(f.n, in)));
+    double hdo = dot(h, out);
+    double sd = hdi + eta * hdo;
+    double dh_dout = eta * eta * hdo / (sd * sd);
+    return (1 - F) * D * Gi * fabs(dh_dout * hdi / dot(f.n, in));
+}
+LJX_HD BsdfSample glass_sample(const Glass &gl, D3 in, const ShadeGeom &g, double u0, double u1, double w) {
+    double eta = dot(g.ng, in) > 0 ? gl.eta : 1 / gl.eta;
+    Frame f = dielectric_frame(g, in);
+    double ax = gl.aniso ? gl.ax : gl.rough * gl.rough;
+    double ay = gl.aniso ? gl.ay : gl.rough * gl.rough;
+    D3 h = to_world(f, sample_vndf(to_local(f, in), ax, ay, u0, u1));
+    if (dot(h, f.n) < 0) h = -h;
+    double hdi = dot(h, in);
+    double F = fresnel_dielectric(hdi, eta);
+    if (w <= F) return BsdfSample{reflect_about(in, h), 0, gl.rough, 1};
+    double hdo2 = 1 - (1 - hdi * hdi) / (eta * eta);
+    if (hdo2 <= 0) return BsdfSample{{0, 0, 0}, 0, 0, 0};
+    if (hdi < 0) h = -h;
+    double hdo = sqrt(hdo2);
+    D3 refr = -in / eta + (fabs(hdi) / eta - hdo) * h;
+    return BsdfSample{refr, eta, gl.rough, 1};
+}
+LJX_HD Glass dielectric_glass(const MatRec &m) {
+    Glass gl;
+    gl.Kr = m.a;
+    gl.Kt = m.b;
+    gl.eta = m.eta;
+    gl.aniso = 0;
+    gl.rough = clamp_rough(m.roughness);
+    gl.ax = gl.ay = gl.rough * gl.rough;
+    return gl;
+}
+
+// ---------------------------------------------------------------- Disney
+// handouts/homework1.tex:93-548 (the reference's Disney materials are stubs).
+LJX_HD void disney_alphas(double rough, double aniso, double &ax, double &ay) {
+    double aspect = sqrt(1 - 0.9 * aniso);
+    ax = fmax(1e-4, rough * rough / aspect);
+    ay = fmax(1e-4, rough * rough * aspect);
+}
+LJX_HD Glass disney_glass(D3 base, double rough, double aniso, double eta) {
+    Glass gl;
+    gl.Kr = base;
+    gl.Kt = vsqrt0(base);
+    gl.eta = eta;
+    gl.aniso = 1;
+    gl.rough = rough;
+    disney_alphas(rough, aniso, gl.ax, gl.ay);
+    return gl;
+}
+LJX_HD D3 tint_color(D3 base) {
+    double l = luminance(base);
+    return l > 0 ? base / l : D3{1, 1, 1};
+}
+LJX_HD D3 disney_diffuse_eval(D3 base, double rough, double subsurface, D3 in, D3 out, const ShadeGeom &g) {
+    if (dot(g.ng, in) < 0 || dot(g.ng, out) < 0) return {0, 0, 0};
+    Frame f = front_frame(g, in);
+    D3 h = normalize(in + out);
+    double ni = fabs(dot(f.n, in)), no = fabs(dot(f.n, out));
+    double ho = fabs(dot(h, out));
+    double fd90 = 0.5 + 2 * rough * ho * ho;
+    double fdi = 1 + (fd90 - 1) * pow5(1 - ni), fdo = 1 + (fd90 - 1) * pow5(1 - no);
+    D3 base_d = base * (fdi * fdo * no / kPi);
+    double fss90 = rough * ho * ho;
+    double fsi = 1 + (fss90 - 1) * pow5(1 - ni), fso = 1 + (fss90 - 1) * pow5(1 - no);
+    D3 ss = base * (1.25 / kPi * (fsi * fso * (1 / (ni + no) - 0.5) + 0.5) * no);
+    return (1 - subsurface) * base_d + subsurface * ss;
+}
+LJX_HD D3 disney_metal_eval(D3 F0, double ax, double ay, D3 in, D3 out, const ShadeGeom &g) {
+    if (dot(g.ng, in) < 0 || dot(g.ng, out) < 0) return {0, 0, 0};
+    Frame f = front_frame(g, in);
+    D3 h = normalize(in + out);
+    double ndi = dot(f.n, in), ndo = dot(f.n, out), ndh = dot(f.n, h);
+    if (ndo <= 0 || ndh <= 0) return {0, 0, 0};
+    D3 Fm = F0 + (1 - F0) * pow5(1 - fabs(dot(h, out)));
+    double D = ggx_aniso_D(to_local(f, h), ax, ay);
+    double G = ggx_aniso_G1(to_local(f, in), ax, ay) * ggx_aniso_G1(to_local(f, out), ax, ay);
+    return Fm * (D * G / (4 * fabs(ndi)));
+}
+LJX_HD double disney_metal_pdf(double ax, double ay, D3 in, D3 out, const ShadeGeom &g) {
+    if (dot(g.ng, in) < 0 || dot(g.ng, out) < 0) return 0;
+    Frame f = front_frame(g, in);
+    D3 h = normalize(in + out);
+    double ndi = dot(f.n, in), ndo = dot(f.n, out), ndh = dot(f.n, h);
+    if (ndo <= 0 || ndh <= 0) return 0;
+    return ggx_aniso_D(to_local(f, h), ax, ay) * ggx_aniso_G1(to_local(f, in), ax, ay) / (4 * fabs(ndi));
+}
+LJX_HD BsdfSample disney_metal_sample(double ax, double ay, double rough, D3 in, const ShadeGeom &g, double u0,
+                                      double u1) {
+    if (dot(g.ng, in) < 0) return BsdfSample{{0, 0, 0}, 0, 0, 0};
+    Frame f = front_frame(g, in);
+    D3 h = to_world(f, sample_vndf(to_local(f, in), ax, ay, u0, u1));
+    return BsdfSample{reflect_about(in, h), 0, rough, 1};
+}
+LJX_HD double cc_alpha(double gloss) { return (1 - gloss) * 0.1 + gloss * 0.001; }
+LJX_HD double cc_D(double hz, double ag) {
+    double a2 = ag * ag;
+    return (a2 - 1) / (kPi * log(a2) * (1 + (a2 - 1) * hz * hz));
+}
+LJX_HD D3 disney_cc_eval(double gloss, D3 in, D3 out, const ShadeGeom &g) {
+    if (dot(g.ng, in) < 0 || dot(g.ng, out) < 0) return {0, 0, 0};
+    Frame f = front_frame(g, in);
+    D3 h = normalize(in + out);
+    double ndi = dot(f.n, in), ndo = dot(f.n, out), ndh = dot(f.n, h);
+    if (ndo <= 0 || ndh <= 0) return {0, 0, 0};
+    const double R0 = 0.04;  // ((1.5 - 1) / (1.5 + 1))^2
+    double Fc = R0 + (1 - R0) * pow5(1 - fabs(dot(h, out)));
+    double Dc = cc_D(to_local(f, h).z, cc_alpha(gloss));
+    double Gc = ggx_aniso_G1(to_local(f, in), 0.25, 0.25) * ggx_aniso_G1(to_local(f, out), 0.25, 0.25);
+    double v = Fc * Dc * Gc / (4 * fabs(ndi));
+    return {v, v, v};
+}
+LJX_HD double disney_cc_pdf(double gloss, D3 in, D3 out, const ShadeGeom &g) {
+    if (dot(g.ng, in) < 0 || dot(g.ng, out) < 0) return 0;
+    Frame f = front_frame(g, in);
+    D3 h = normalize(in + out);
+    double ndo = dot(f.n, out), ndh = dot(f.n, h);
+    if (ndo <= 0 || ndh <= 0) return 0;
+    return cc_D(to_local(f, h).z, cc_alpha(gloss)) * fabs(ndh) / (4 * fabs(dot(h, out)));
+}
+LJX_HD BsdfSample disney_cc_sample(double gloss, D3 in, const ShadeGeom &g, double u0, double u1) {
+    if (dot(g.ng, in) < 0) return BsdfSample{{0, 0, 0}, 0, 0, 0};
+    Frame f = front_frame(g, in);
+    double ag = cc_alpha(gloss);
+    double a2 = ag * ag;
+    double ch = sqrt(fmax(0.0, (1 - pow(a2, 1 - u0)) / (1 - a2)));
+    double sh = sqrt(fmax(0.0, 1 - ch * ch));
+    double s, c;
+    sincos(2 * kPi * u1, &s, &c);
+    D3 h = to_world(f, D3{sh * c, sh * s, ch});
+    return BsdfSample{reflect_about(in, h), 0, sqrt(ag), 1};
+}
+LJX_HD D3 disney_sheen_eval(D3 base, double tint, D3 in, D3 out, const ShadeGeom &g) {
+    if (dot(g.ng, in) < 0 || dot(g.ng, out) < 0) return {0, 0, 0};
+    Frame f = front_frame(g, in);
+    D3 h = normalize(in + out);
+    D3 C = (1 - tint) + tint * tint_color(base);
+    return C * (pow5(1 - fabs(dot(h, out))) * fabs(dot(f.n, out)));
+}
+struct DisneyLobes {
+    double w_diffuse, w_sheen, w_metal, w_cc, w_glass;
+};
+LJX_HD DisneyLobes disney_lobes(const MatRec &m) {
+    DisneyLobes l;
+    l.w_diffuse = (1 - m.specular_transmission) * (1 - m.metallic);
+    l.w_sheen = (1 - m.metallic) * m.sheen;
+    l.w_metal = 1 - m.specular_transmission * (1 - m.metallic);
+    l.w_cc = 0.25 * m.clearcoat;
+    l.w_glass = (1 - m.metallic) * m.specular_transmission;
+    return l;
+}
+LJX_HD D3 disney_F0(const MatRec &m) {
+    double R0 = (m.eta - 1) * (m.eta - 1) / ((m.eta + 1) * (m.eta + 1));
+    D3 Ks = (1 - m.specular_tint) + m.specular_tint * tint_color(m.a);
+    return m.specular * R0 * (1 - m.metallic) * Ks + m.metallic * m.a;
+}
+LJX_HD D3 disney_bsdf_eval(const MatRec &m, D3 in, D3 out, const ShadeGeom &g, bool to_view) {
+    double ax, ay;
+    disney_alphas(m.roughness, m.anisotropic, ax, ay);
+    Glass gl = disney_glass(m.a, m.roughness, m.anisotropic, m.eta);
+    DisneyLobes w = disney_lobes(m);
+    if (dot(g.ng, in) <= 0) return w.w_glass * glass_eval(gl, in, out, g, to_view);
+    D3 r = w.w_diffuse * disney_diffuse_eval(m.a, m.roughness, m.subsurface, in, out, g);
+    r = r + w.w_sheen * disney_sheen_eval(m.a, m.sheen_tint, in, out, g);
+    r = r + w.w_metal * disney_metal_eval(disney_F0(m), ax, ay, in, out, g);
+    r = r + w.w_cc * disney_cc_eval(m.clearcoat_gloss, in, out, g);
+    r = r + w.w_glass * glass_eval(gl, in, out, g, to_view);
+    return r;
+}
+LJX_HD void disney_probs(const MatRec &m, bool inside, double p[4]) {
+    if (inside) {
+        p[0] = p[1] = p[3] = 0;
+        p[2] = 1;
+        return;
+    }
+    DisneyLobes w = disney_lobes(m);
+    double tot = w.w_diffuse + w.w_metal + w.w_glass + w.w_cc;
+    p[0] = w.w_diffuse / tot;
+    p[1] = w.w_metal / tot;
+    p[2] = w.w_glass / tot;
+    p[3] = w.w_cc / tot;
+}
+LJX_HD double disney_bsdf_pdf(const MatRec &m, D3 in, D3 out, const ShadeGeom &g) {
+    double ax, ay;
+    disney_alphas(m.roughness, m.anisotropic, ax, ay);
+    Glass gl = disney_glass(m.a, m.roughness, m.anisotropic, m.eta);
+    bool inside = dot(g.ng, in) <= 0;
+    double p[4];
+    disney_probs(m, inside, p);
+    if (inside) return glass_pdf(gl, in, out, g);
+    double r = 0;
+    if (p[0] > 0) r += p[0] * cos_pdf(in, out, g);
+    if (p[1] > 0) r += p[1] * disney_metal_pdf(ax, ay, in, out, g);
+    if (p[2] > 0) r += p[2] * glass_pdf(gl, in, out, g);
+    if (p[3] > 0) r += p[3] * disney_cc_pdf(m.clearcoat_gloss, in, out, g);
+    return r;
+}
+LJX_HD BsdfSample disney_bsdf_sample(const MatRec &m, D3 in, const ShadeGeom &g, double u0, double u1,
+                                     double w) {
+    double ax, ay;
+    disney_alphas(m.roughness, m.anisotropic, ax, ay);
+    Glass gl = disney_glass(m.a, m.roughness, m.anisotropic, m.eta);
+    bool inside = dot(g.ng, in) <= 0;
+    double p[4];
+    disney_probs(m, inside, p);
+    if (inside) return glass_sample(gl, in, g, u0, u1, w);
+    double c0 = p[0], c1 = c0 + p[1], c2 = c1 + p[2];
+    if (w < c0) return cos_sample(in, g, u0, u1, m.roughness);
+    if (w < c1) return disney_metal_sample(ax, ay, m.roughness, in, g, u0, u1);
+    if (w < c2) return glass_sample(gl, in, g, u0, u1, (w - c1) / p[2]);
+    return disney_cc_sample(m.clearcoat_gloss, in, g, u0, u1);
+}
+
+// -------------------------------------------------- material dispatch (MT)
+// MT is the material type tag of include/lajolla_amd.h; MT = -1 dispatches
+// at run time.  Shading kernels are instantiated per type so that each
+// sorted bucket runs straight-line code.
+template <int MT>
+LJX_HD D3 bsdf_eval(const MatRec &m, D3 in, D3 out, const ShadeGeom &g, bool to_view = false) {
+    const int t = MT >= 0 ? MT : m.type;
+    switch (t) {
+    case 0: return lambert_eval(m.a, in, out, g);
+    case 1: return plastic_eval(m, in, out, g);
+    case 2: return glass_eval(dielectric_glass(m), in, out, g, to_view);
+    case 3: return disney_diffuse_eval(m.a, m.roughness, m.subsurface, in, out, g);
+    case 4: {
+        double ax, ay;
+        disney_alphas(m.roughness, m.anisotropic, ax, ay);
+        return disney_metal_eval(m.a, ax, ay, in, out, g);
+    }
+    case 5: return glass_eval(disney_glass(m.a, m.roughness, m.anisotropic, m.eta), in, out, g, to_view);
+    case 6: return disney_cc_eval(m.clearcoat_gloss, in, out, g);
+    case 7: return disney_sheen_eval(m.a, m.sheen_tint, in, out, g);
+    default: return disney_bsdf_eval(m, in, out, g, to_view);
+    }
+}
+template <int MT>
+LJX_HD double bsdf_pdf(const MatRec &m, D3 in, D3 out, const ShadeGeom &g) {
+    const int t = MT >= 0 ? MT : m.type;
+    switch (t) {
+    case 0: return cos_pdf(in, out, g);
+    case 1: return plastic_pdf(m, in, out, g);
+    case 2: return glass_pdf(dielectric_glass(m), in, out, g);
+    case 3: return cos_pdf(in, out, g);
+    case 4: {
+        double ax, ay;
+        disney_alphas(m.roughness, m.anisotropic, ax, ay);
+        return disney_metal_pdf(ax, ay, in, out, g);
+    }
+    case 5: return glass_pdf(disney_glass(m.a, m.roughness, m.anisotropic, m.eta), in, out, g);
+    case 6: return disney_cc_pdf(m.clearcoat_gloss, in, out, g);
+    case 7: return cos_pdf(in, out, g);
+    default: return disney_bsdf_pdf(m, in, out, g);
+    }
+}
+template <int MT>
+LJX_HD BsdfSample bsdf_sample(const MatRec &m, D3 in, const ShadeGeom &g, double u0, double u1, double w) {
+    const int t = MT >= 0 ? MT : m.type;
+    switch (t) {
+    case 0: return cos_sample(in, g, u0, u1, 1.0);
+    case 1: return plastic_sample(m, in, g, u0, u1, w);
+    case 2: return glass_sample(dielectric_glass(m), in, g, u0, u1, w);
+    case 3: return cos_sample(in, g, u0, u1, m.roughness);
+    case 4: {
+        double ax, ay;
+        disney_alphas(m.roughness, m.anisotropic, ax, ay);
+        return disney_metal_sample(ax, ay, m.roughness, in, g, u0, u1);
+    }
+    case 5: return glass_sample(disney_glass(m.a, m.roughness, m.anisotropic, m.eta), in, g, u0, u1, w);
+    case 6: return disney_cc_sample(m.clearcoat_gloss, in, g, u0, u1);
+    case 7: return cos_sample(in, g, u0, u1, 1.0);
+    default: return disney_bsdf_sample(m, in, g, u0, u1, w);
+    }
+}
+
+// ----------------------------------------------------------------- phase
+// src/phase_functions/isotropic.inl:1-15, henyeygreenstein.inl:1-48
+LJX_HD double phase_value(const MediumRec &m, D3 in, D3 out) {
+    if (m.phase_type == 0) return kInvFourPi;
+    double g = m.g;
+    return kInvFourPi * (1 - g * g) / pow(1 + g * g + 2 * g * dot(in, out), 1.5);
+}
+LJX_HD D3 phase_sample(const MediumRec &m, D3 in, double u0, double u1) {
+    double s, c;
+    if (m.phase_type == 0 || fabs(m.g) < 1e-3) {
+        double z = 1 - 2 * u0;
+        double r = sqrt(fmax(0.0, 1 - z * z));
+        sincos(2 * kPi * u1, &s, &c);
+        return {r * c, r * s, z};
+    }
+    double g = m.g;
+    double tmp = (g * g - 1) / (2 * u0 * g - (g + 1));
+    double ce = (tmp * tmp - (1 + g * g)) / (2 * g);
+    double se = sqrt(fmax(1 - ce * ce, 0.0));
+    sincos(2 * kPi * u1, &s, &c);
+    return to_world(frame_from_n(in), D3{se * c, se * s, ce});
+}
+
+// ----------------------------------------------------------------- camera
+LJX_HD D3 xform_point(const double *m, D3 p) {
+    double t0 = m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3];
+    double t1 = m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7];
+    double t2 = m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11];
+    double t3 = m[12] * p.x + m[13] * p.y + m[14] * p.z + m[15];
+    double iw = 1 / t3;
+    return {t0 * iw, t1 * iw, t2 * iw};
+}
+LJX_HD D3 xform_vector(const double *m, D3 v) {
+    return {m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+            m[8] * v.x + m[9] * v.y + m[10] * v.z};
+}
+LJX_HD D3 xform3(const double *m, D3 v) {
+    return {m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+            m[6] * v.x + m[7] * v.y + m[8] * v.z};
+}
+// Pixel filters (src/filters/{box,tent,gaussian}.inl).
+LJX_HD void filter_sample(int type, double param, double u0, double u1, double &ox, double &oy) {
+    if (type == 0) {
+        ox = (2 * u0 - 1) * (param / 2);
+        oy = (2 * u1 - 1) * (param / 2);
+    } else if (type == 1) {
+        double h = param / 2;
+        ox = u0 < 0.5 ? h * (sqrt(2 * u0) - 1) : h * (1 - sqrt(1 - 2 * (u0 - 0.5)));
+        oy = u1 < 0.5 ? h * (sqrt(2 * u1) - 1) : h * (1 - sqrt(1 - 2 * (u1 - 0.5)));
+    } else {
+        double r = param * sqrt(-2 * log(fmax(u0, 1e-8)));
+        double s, c;
+        sincos(2 * kPi * u1, &s, &c);
+        ox = r * c;
+        oy = r * s;
+    }
+}
+// sample_primary (src/camera.cpp:23-47).
+LJX_HD void sample_primary(const CameraRec &c, double sx, double sy, D3 &org, D3 &dir) {
+    double px = sx * c.width, py = sy * c.height;
+    double fx = floor(px), fy = floor(py);
+    double ox, oy;
+    filter_sample(c.filter_type, c.filter_param, px - fx, py - fy, ox, oy);
+    D3 rp{(fx + 0.5 + ox) / c.width, (fy + 0.5 + oy) / c.height, 0};
+    D3 pt = xform_point(c.sample_to_cam, rp);
+    D3 d = normalize(pt);
+    org = xform_point(c.cam_to_world, D3{0, 0, 0});
+    dir = normalize(xform_vector(c.cam_to_world, d));
+}
+
+}  // namespace ljx

@@ -1,0 +1,413 @@
+// Wavefront path-tracing kernels (surface transport).
+//
+// One bounce of lajolla's path_tracing (src/path_tracing.h:7-325) is split
+// into stages that each run over a compacted SoA queue of path ids:
+//
+//   k_raygen      sample_primary + the two screen-jitter draws
+//   k_extend      closest-hit traversal; appends each path to the bucket of
+//                 the hit material's type (material sort) or to MISS
+//   k_shade<MT>   one kernel per material type: emission + MIS of the hit
+//                 reached by the previous BSDF sample, Russian roulette,
+//                 throughput update, then NEE (light pick, point, shadow ray
+//                 with its pending contribution) and BSDF sampling
+//   k_miss        environment term of escaped paths
+//   k_shadow      any-hit traversal; adds the pending NEE contribution
+//   k_accumulate  per-pixel sum of the pass's samples in sample order
+//
+// Each path draws from its own PCG stream (stream = sample * W * H + pixel)
+// in the reference's per-path order, so results do not depend on launch
+// shape or queue order.
+#pragma once
+
+#include "ljx_shade.h"
+
+namespace ljx {
+
+constexpr int kNumMatTypes = 9;
+constexpr int kBucketMiss = 9;
+constexpr int kBucketNoMat = 10;
+constexpr int kNumBuckets = 11;
+// counter slots
+constexpr int C_EXT = 0;      // C_EXT + (0|1): extension queues
+constexpr int C_SH = 2;       // shadow queue
+constexpr int C_BUCKET = 4;   // C_BUCKET + bucket
+constexpr int C_TOTAL = 16;
+// 64-bit statistics slots
+constexpr int S_EXT_RAYS = 0, S_SH_RAYS = 1, S_NODES = 2, S_PRIMS = 3, S_SH_NODES = 4, S_SH_PRIMS = 5,
+              S_NUM = 8;
+
+struct PathBufs {
+    uint64_t *rng;
+    float4 *tp;       // throughput rgb, eta_scale
+    float4 *rad;      // radiance rgb
+    float4 *prev;     // f (with cosine) of the last BSDF sample, its solid-angle pdf
+    int *depth;       // index b of the vertex a path's current ray will hit (0: primary)
+    float4 *ray_o;    // origin, tnear
+    float4 *ray_d;    // direction, tfar
+    float4 *hit;      // t, u, v, slot (int bits; -1: miss)
+    uint32_t *extq0, *extq1;
+    uint32_t *bucket; // kNumBuckets * capacity
+    uint32_t *shq;
+    float4 *sh_o, *sh_d, *sh_c;
+    int *ctr;
+    unsigned long long *stat;
+    int *spill;
+    double *accum;    // W * H * 3
+    int capacity;
+    // image / pass
+    int W, H, ntx8;
+    int pass_s0, pass_ns, pass_q0, pass_nq;
+    int spp, max_depth, rr_depth;
+    int volpath;
+    int max_null;
+    uint64_t seed;
+};
+
+__device__ __forceinline__ void slot_pixel(const PathBufs &P, int i, int &s, int &x, int &y, bool &valid) {
+    s = P.pass_s0 + i / P.pass_nq;
+    int q = P.pass_q0 + i % P.pass_nq;
+    int tile = q >> 6, w = q & 63;
+    x = (tile % P.ntx8) * 8 + (w & 7);
+    y = (tile / P.ntx8) * 8 + (w >> 3);
+    valid = x < P.W && y < P.H;
+}
+__device__ __forceinline__ uint64_t path_stream(const PathBufs &P, int s, int x, int y) {
+    return uint64_t(s) * uint64_t(P.W) * uint64_t(P.H) + uint64_t(y) * P.W + x;
+}
+
+// Counter resets folded into kernels that do not read those counters.
+__global__ void k_reset_counters(int *ctr, int first, int count) {
+    int i = threadIdx.x;
+    if (i < count) ctr[first + i] = 0;
+}
+
+__global__ void k_fill_f64(double *p, size_t n) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0.0;
+}
+
+__global__ void k_raygen(DeviceScene S, PathBufs P) {
+    int n = P.pass_ns * P.pass_nq;
+    int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        int i = base + threadIdx.x;
+        int s = 0, x = 0, y = 0;
+        bool valid = false;
+        if (i < n) slot_pixel(P, i, s, x, y, valid);
+        if (valid) {
+            uint64_t stream = path_stream(P, s, x, y);
+            uint64_t inc = pcg_inc(stream);
+            uint64_t st = pcg_init_state(stream, P.seed);
+            double r0 = pcg_double(st, inc), r1 = pcg_double(st, inc);
+            D3 o, d;
+            sample_primary(S.camera, (x + r0) / P.W, (y + r1) / P.H, o, d);
+            P.rng[i] = st;
+            P.ray_o[i] = make_float4(float(o.x), float(o.y), float(o.z), 0.0f);
+            P.ray_d[i] = make_float4(float(d.x), float(d.y), float(d.z), __builtin_huge_valf());
+            P.tp[i] = make_float4(1.f, 1.f, 1.f, 1.f);
+            P.prev[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            P.depth[i] = 0;
+        }
+        if (i < n) P.rad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        int k = wave_append(valid, &P.ctr[C_EXT + 0]);
+        if (k >= 0) P.extq0[k] = uint32_t(i);
+    }
+}
+
+// Closest-hit traversal over the extension queue, then material bucketing.
+template <bool STATS>
+__global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int cur) {
+    __shared__ int stack[kStackLds * 256];
+    const uint32_t *q = cur ? P.extq1 : P.extq0;
+    int n = P.ctr[C_EXT + cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        P.ctr[C_EXT + (cur ^ 1)] = 0;
+        P.ctr[C_SH] = 0;
+        atomicAdd(&P.stat[S_EXT_RAYS], (unsigned long long)n);
+    }
+    int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    int stride = gridDim.x * blockDim.x;
+    int *spill = P.spill + gtid;
+    uint32_t nn = 0, np = 0;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        int i = base + threadIdx.x;
+        int bucket = -1;
+        uint32_t pid = 0;
+        if (i < n) {
+            pid = q[i];
+            float4 o = P.ray_o[pid], d = P.ray_d[pid];
+            TraceHit h;
+            bool hit = S.num_nodes > 0 &&
+                       trace_ray<false, STATS>(S.nodes, S.prims, o.x, o.y, o.z, d.x, d.y, d.z, o.w, d.w,
+                                               stack + threadIdx.x, blockDim.x, spill, stride, h, nn, np);
+            if (hit) {
+                P.hit[pid] = make_float4(h.t, h.u, h.v, __int_as_float(h.slot));
+                int gid = __float_as_int(S.prims[3 * h.slot].w);
+                int mid = S.shapes[S.prim_shape[gid]].material_id;
+                bucket = mid >= 0 ? S.materials[mid].type : kBucketNoMat;
+            } else {
+                P.hit[pid] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+                bucket = kBucketMiss;
+            }
+        }
+        // one append per distinct bucket present in the wave
+        uint64_t pending = __ballot(bucket >= 0);
+        while (pending) {
+            int leader = __ffsll((unsigned long long)pending) - 1;
+            int b = __shfl(bucket, leader);
+            bool mine = bucket == b;
+            int k = wave_append(mine, &P.ctr[C_BUCKET + b]);
+            if (mine) P.bucket[size_t(b) * P.capacity + k] = pid;
+            pending &= ~__ballot(mine);
+        }
+    }
+    if (STATS) {
+        atomicAdd(&P.stat[S_NODES], (unsigned long long)nn);
+        atomicAdd(&P.stat[S_PRIMS], (unsigned long long)np);
+    }
+}
+
+// Pending NEE contribution and the next ray of one path vertex.
+template <int MT>
+__device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBufs &P, uint32_t pid, bool &push_ext,
+                                              bool &push_sh, float4 &sh_o, float4 &sh_d, float4 &sh_c) {
+    float4 h = P.hit[pid];
+    float4 ro = P.ray_o[pid], rd = P.ray_d[pid];
+    D3 org = f3d(ro), dir = f3d(rd);
+    int b = P.depth[pid];
+    uint64_t st = P.rng[pid];
+    uint64_t inc = 0;
+    {
+        // recover the stream of this slot
+        int s, x, y;
+        bool valid;
+        slot_pixel(P, int(pid), s, x, y, valid);
+        inc = pcg_inc(path_stream(P, s, x, y));
+    }
+    float4 tp4 = P.tp[pid];
+    D3 tp{tp4.x, tp4.y, tp4.z};
+    double eta_scale = tp4.w;
+    float4 rad4 = P.rad[pid];
+    D3 rad{rad4.x, rad4.y, rad4.z};
+    Vertex v = make_vertex(S, __float_as_int(h.w), h.x, h.y, h.z, org, dir);
+    const MatRec &mat = S.materials[v.material_id];
+    bool alive = true;
+    push_ext = false;
+    push_sh = false;
+    if (b == 0) {
+        // path_tracing.h:58-61
+        if (v.area_light_id >= 0)
+            rad = rad + tp * light_emission(S.lights[v.area_light_id], -dir, LightPoint{v.pos, v.g.ng});
+    } else {
+        // path_tracing.h:243-322 for the vertex reached by the previous sample
+        float4 pv = P.prev[pid];
+        D3 f{pv.x, pv.y, pv.z};
+        double pdf = pv.w;
+        double G = fabs(dot(dir, v.g.ng)) / dist2(v.pos, org);
+        double p2 = pdf * G;
+        if (v.area_light_id >= 0) {
+            const LightRec &l = S.lights[v.area_light_id];
+            LightPoint lp{v.pos, v.g.ng};
+            D3 L = light_emission(l, -dir, lp);
+            D3 C2 = G * f * L;
+            double p1 = S.light_pmf[v.area_light_id] * pdf_light_point(S, l, lp, org);
+            double w2 = (p2 * p2) / (p1 * p1 + p2 * p2);
+            C2 = C2 / p2;
+            rad = rad + tp * C2 * w2;
+        }
+        double rr = 1;
+        if (b + 1 >= P.rr_depth) {
+            rr = fmin(maxc((1 / eta_scale) * tp), 0.95);
+            if (pcg_double(st, inc) > rr) alive = false;
+        }
+        if (alive) tp = tp * (G * f) / (p2 * rr);
+    }
+    if (alive && !(P.max_depth == -1 || b + 3 <= P.max_depth + 1)) alive = false;
+    if (alive && S.num_lights == 0) {
+        // the reference asserts on an empty light table (src/table_dist.cpp:29); end the path
+        alive = false;
+    }
+    if (alive) {
+        // next-event estimation (path_tracing.h:96-207)
+        double lu0 = pcg_double(st, inc), lu1 = pcg_double(st, inc);
+        double lw = pcg_double(st, inc), sw = pcg_double(st, inc);
+        int lid = sample_cdf(S.light_cdf, S.num_lights, lw);
+        const LightRec &l = S.lights[lid];
+        LightPoint lp = sample_light_point(S, l, v.pos, lu0, lu1, sw);
+        D3 dir_light;
+        double G = 0, tfar;
+        if (l.type == 0) {
+            dir_light = normalize(lp.pos - v.pos);
+            double dl = sqrt(dist2(lp.pos, v.pos));
+            tfar = (1 - S.eps) * dl;
+            G = fmax(-dot(dir_light, lp.n), 0.0) / dist2(lp.pos, v.pos);
+        } else {
+            dir_light = -lp.n;
+            tfar = __builtin_huge_val();
+            G = 1;
+        }
+        double p1 = S.light_pmf[lid] * pdf_light_point(S, l, lp, v.pos);
+        D3 dir_view = -dir;
+        if (G > 0 && p1 > 0) {
+            D3 f = bsdf_eval<MT>(mat, dir_view, dir_light, v.g);
+            D3 L = light_emission(l, -dir_light, lp);
+            D3 C1 = G * f * L;
+            double p2 = bsdf_pdf<MT>(mat, dir_view, dir_light, v.g) * G;
+            double w1 = (p1 * p1) / (p1 * p1 + p2 * p2);
+            C1 = C1 / p1;
+            D3 c = tp * C1 * w1;
+            if (c.x != 0 || c.y != 0 || c.z != 0) {
+                push_sh = true;
+                sh_o = make_float4(float(v.pos.x), float(v.pos.y), float(v.pos.z), float(S.eps));
+                sh_d = make_float4(float(dir_light.x), float(dir_light.y), float(dir_light.z), float(tfar));
+                sh_c = make_float4(float(c.x), float(c.y), float(c.z), __int_as_float(int(pid)));
+            }
+        }
+        // BSDF sampling (path_tracing.h:209-259)
+        double bu0 = pcg_double(st, inc), bu1 = pcg_double(st, inc), bw = pcg_double(st, inc);
+        BsdfSample bs = bsdf_sample<MT>(mat, dir_view, v.g, bu0, bu1, bw);
+        if (!bs.valid) {
+            alive = false;
+        } else {
+            if (bs.eta != 0) eta_scale /= (bs.eta * bs.eta);
+            D3 f = bsdf_eval<MT>(mat, dir_view, bs.dir, v.g);
+            double pdf = bsdf_pdf<MT>(mat, dir_view, bs.dir, v.g);
+            if (pdf <= 0) {
+                alive = false;
+            } else {
+                P.prev[pid] = make_float4(float(f.x), float(f.y), float(f.z), float(pdf));
+                P.ray_o[pid] = make_float4(float(v.pos.x), float(v.pos.y), float(v.pos.z), float(S.eps));
+                P.ray_d[pid] = make_float4(float(bs.dir.x), float(bs.dir.y), float(bs.dir.z), __builtin_huge_valf());
+                P.depth[pid] = b + 1;
+                push_ext = true;
+            }
+        }
+    }
+    P.rng[pid] = st;
+    P.tp[pid] = make_float4(float(tp.x), float(tp.y), float(tp.z), float(eta_scale));
+    P.rad[pid] = make_float4(float(rad.x), float(rad.y), float(rad.z), 0.f);
+}
+
+template <int MT>
+__global__ __launch_bounds__(256) void k_shade(DeviceScene S, PathBufs P, int cur) {
+    int n = P.ctr[C_BUCKET + MT];
+    const uint32_t *q = P.bucket + size_t(MT) * P.capacity;
+    uint32_t *next = cur ? P.extq0 : P.extq1;
+    int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        int i = base + threadIdx.x;
+        bool pe = false, ps = false;
+        float4 so, sd, sc;
+        uint32_t pid = 0;
+        if (i < n) {
+            pid = q[i];
+            shade_surface<MT>(S, P, pid, pe, ps, so, sd, sc);
+        }
+        int k = wave_append(pe, &P.ctr[C_EXT + (cur ^ 1)]);
+        if (k >= 0) next[k] = pid;
+        int j = wave_append(ps, &P.ctr[C_SH]);
+        if (j >= 0) {
+            P.shq[j] = pid;
+            P.sh_o[j] = so;
+            P.sh_d[j] = sd;
+            P.sh_c[j] = sc;
+        }
+    }
+}
+
+// Escaped paths: environment emission with MIS (path_tracing.h:17-28, 284-302).
+__global__ __launch_bounds__(256) void k_miss(DeviceScene S, PathBufs P) {
+    int n = P.ctr[C_BUCKET + kBucketMiss];
+    const uint32_t *q = P.bucket + size_t(kBucketMiss) * P.capacity;
+    if (S.envmap_light_id < 0) return;
+    const LightRec &env = S.lights[S.envmap_light_id];
+    int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint32_t pid = q[i];
+        float4 rd = P.ray_d[pid];
+        D3 dir = f3d(rd);
+        float4 tp4 = P.tp[pid];
+        D3 tp{tp4.x, tp4.y, tp4.z};
+        float4 r4 = P.rad[pid];
+        D3 rad{r4.x, r4.y, r4.z};
+        D3 L = light_emission(env, -dir, LightPoint{{0, 0, 0}, {0, 0, 0}});
+        if (P.depth[pid] == 0) {
+            rad = rad + tp * L;
+        } else {
+            float4 pv = P.prev[pid];
+            D3 f{pv.x, pv.y, pv.z};
+            double p2 = pv.w;  // G = 1
+            D3 C2 = f * L;
+            LightPoint lp{{0, 0, 0}, -dir};
+            double p1 = S.light_pmf[S.envmap_light_id] * pdf_light_point(S, env, lp, D3{0, 0, 0});
+            double w2 = (p2 * p2) / (p1 * p1 + p2 * p2);
+            C2 = C2 / p2;
+            rad = rad + tp * C2 * w2;
+        }
+        P.rad[pid] = make_float4(float(rad.x), float(rad.y), float(rad.z), 0.f);
+    }
+}
+
+template <bool STATS>
+__global__ __launch_bounds__(256) void k_shadow(DeviceScene S, PathBufs P) {
+    __shared__ int stack[kStackLds * 256];
+    int n = P.ctr[C_SH];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&P.stat[S_SH_RAYS], (unsigned long long)n);
+        for (int b = 0; b < kNumBuckets; b++) P.ctr[C_BUCKET + b] = 0;
+    }
+    int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    int stride = gridDim.x * blockDim.x;
+    int *spill = P.spill + gtid;
+    uint32_t nn = 0, np = 0;
+    for (int i = gtid; i < n; i += stride) {
+        float4 o = P.sh_o[i], d = P.sh_d[i];
+        TraceHit h;
+        bool occ = S.num_nodes > 0 &&
+                   trace_ray<true, STATS>(S.nodes, S.prims, o.x, o.y, o.z, d.x, d.y, d.z, o.w, d.w,
+                                          stack + threadIdx.x, blockDim.x, spill, stride, h, nn, np);
+        if (!occ) {
+            float4 c = P.sh_c[i];
+            int pid = __float_as_int(c.w);
+            float4 r = P.rad[pid];
+            P.rad[pid] = make_float4(r.x + c.x, r.y + c.y, r.z + c.z, 0.f);
+        }
+    }
+    if (STATS) {
+        atomicAdd(&P.stat[S_SH_NODES], (unsigned long long)nn);
+        atomicAdd(&P.stat[S_SH_PRIMS], (unsigned long long)np);
+    }
+}
+
+// render.cpp:89-94 (path) and :136-145 (volpath drops non-finite samples).
+__global__ void k_accumulate(PathBufs P) {
+    int nq = P.pass_nq;
+    int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nq; j += stride) {
+        int s, x, y;
+        bool valid;
+        slot_pixel(P, j, s, x, y, valid);
+        if (!valid) continue;
+        double r = 0, g = 0, b = 0;
+        for (int k = 0; k < P.pass_ns; k++) {
+            float4 v = P.rad[k * nq + j];
+            if (P.volpath && !(isfinite(v.x) && isfinite(v.y) && isfinite(v.z))) continue;
+            r += v.x;
+            g += v.y;
+            b += v.z;
+        }
+        size_t o = (size_t(y) * P.W + x) * 3;
+        P.accum[o] += r;
+        P.accum[o + 1] += g;
+        P.accum[o + 2] += b;
+    }
+}
+
+__global__ void k_resolve(const double *accum, float *out, size_t n, double inv_spp) {
+    size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = float(accum[i] * inv_spp);
+}
+
+}  // namespace ljx

@@ -1,0 +1,1071 @@
+// C ABI of the MI355X path-tracing core (include/lajolla_amd.h): scene
+// upload, BVH build, the wavefront render loop and the parity entry points.
+#include "../../include/lajolla_amd.h"
+#include "ljx_bvh.h"
+#include "ljx_kernels.h"
+#include "ljx_vol.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace ljx;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct LjxFail {
+    int code;
+    std::string msg;
+};
+
+#define HIPX(call)                                                                              \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            throw LjxFail{e_ == hipErrorOutOfMemory ? LJX_ERR_OUT_OF_MEMORY : LJX_ERR_HIP,      \
+                          std::string(#call) + ": " + hipGetErrorString(e_)};                   \
+    } while (0)
+
+void fail(int code, const std::string &msg) { throw LjxFail{code, msg}; }
+
+template <class F>
+int guarded(F f) {
+    try {
+        f();
+        return LJX_OK;
+    } catch (const LjxFail &e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return LJX_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return LJX_ERR_INVALID_ARGUMENT;
+    }
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+// ---------------------------------------------------------------- matrices
+void mat_mul4(const double *a, const double *b, double *r) {
+    double t[16];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            double s = 0;
+            for (int k = 0; k < 4; k++) s += a[i * 4 + k] * b[k * 4 + j];
+            t[i * 4 + j] = s;
+        }
+    std::memcpy(r, t, sizeof(t));
+}
+// Gauss-Jordan elimination with partial pivoting.
+bool mat_inv4(const double *m, double *out) {
+    double a[4][8];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 8; j++) a[i][j] = j < 4 ? m[i * 4 + j] : (j - 4 == i ? 1.0 : 0.0);
+    for (int c = 0; c < 4; c++) {
+        int piv = c;
+        for (int r = c + 1; r < 4; r++)
+            if (std::fabs(a[r][c]) > std::fabs(a[piv][c])) piv = r;
+        if (a[piv][c] == 0) return false;
+        if (piv != c)
+            for (int j = 0; j < 8; j++) std::swap(a[c][j], a[piv][j]);
+        double inv = 1.0 / a[c][c];
+        for (int j = 0; j < 8; j++) a[c][j] *= inv;
+        for (int r = 0; r < 4; r++) {
+            if (r == c) continue;
+            double f = a[r][c];
+            if (f == 0) continue;
+            for (int j = 0; j < 8; j++) a[r][j] -= f * a[c][j];
+        }
+    }
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) out[i * 4 + j] = a[i][j + 4];
+    return true;
+}
+
+double luminance_h(const double *c) { return c[0] * 0.212671 + c[1] * 0.715160 + c[2] * 0.072169; }
+
+// TableDist1D construction (src/table_dist.cpp:3-24).
+void make_dist(const std::vector<double> &f, std::vector<double> &pmf, std::vector<double> &cdf) {
+    size_t n = f.size();
+    pmf = f;
+    cdf.assign(n + 1, 0.0);
+    for (size_t i = 0; i < n; i++) cdf[i + 1] = cdf[i] + pmf[i];
+    double total = cdf[n];
+    if (total > 0) {
+        for (size_t i = 0; i < n; i++) {
+            pmf[i] /= total;
+            cdf[i] /= total;
+        }
+    } else {
+        for (size_t i = 0; i < n; i++) {
+            pmf[i] = 1.0 / double(n);
+            cdf[i] = double(i) / double(n);
+        }
+        cdf[n] = 1;
+    }
+}
+
+// --------------------------------------------------------------- the scene
+struct RenderCtx {
+    int capacity = 0;
+    int trace_grid = 0;
+    PathBufs P{};
+    std::vector<DevBuf> bufs;
+};
+
+struct Scene {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DeviceScene ds{};
+    std::vector<DevBuf> bufs;
+    ljx_render_options options{};
+    int width = 0, height = 0;
+    int num_cu = 0;
+    int num_materials = 0;
+    unsigned mat_types_present = 0;
+    bool has_media = false;
+    double bound_center[3] = {0, 0, 0};
+    double bound_radius = 0;
+    int bvh_depth = 0;
+    double build_ms = 0;
+    std::unique_ptr<RenderCtx> rc;
+    int rc_alloc_W = 0, rc_alloc_H = 0;
+
+    ~Scene() {
+        if (device >= 0) hipSetDevice(device);
+        if (stream) hipStreamSynchronize(stream);
+        release_ctx();
+        for (auto &b : bufs)
+            if (b.p) hipFree(b.p);
+        if (stream) hipStreamDestroy(stream);
+    }
+    void release_ctx() {
+        if (rc) {
+            for (auto &b : rc->bufs)
+                if (b.p) hipFree(b.p);
+            rc.reset();
+        }
+    }
+    template <class T>
+    T *upload(const std::vector<T> &v, std::vector<DevBuf> *into = nullptr) {
+        size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
+        void *p = nullptr;
+        HIPX(hipMalloc(&p, bytes));
+        (into ? *into : bufs).push_back(DevBuf{p, bytes});
+        if (!v.empty()) HIPX(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return static_cast<T *>(p);
+    }
+    template <class T>
+    T *alloc(size_t count, std::vector<DevBuf> &into) {
+        size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+        void *p = nullptr;
+        HIPX(hipMalloc(&p, bytes));
+        into.push_back(DevBuf{p, bytes});
+        return static_cast<T *>(p);
+    }
+};
+
+void validate(const ljx_scene_desc *d) {
+    if (!d) fail(LJX_ERR_INVALID_ARGUMENT, "scene descriptor is NULL");
+    const ljx_camera_desc &c = d->camera;
+    if (c.width <= 0 || c.height <= 0) fail(LJX_ERR_INVALID_ARGUMENT, "camera width/height must be positive");
+    if (c.filter_type > 2) fail(LJX_ERR_INVALID_ARGUMENT, "unknown filter type");
+    if (d->num_shapes < 0 || d->num_materials < 0 || d->num_lights < 0 || d->num_media < 0)
+        fail(LJX_ERR_INVALID_ARGUMENT, "negative element count");
+    if ((d->num_shapes && !d->shapes) || (d->num_materials && !d->materials) || (d->num_lights && !d->lights) ||
+        (d->num_media && !d->media))
+        fail(LJX_ERR_INVALID_ARGUMENT, "NULL array with a nonzero count");
+    if (c.medium_id < -1 || c.medium_id >= d->num_media) fail(LJX_ERR_INVALID_ARGUMENT, "camera medium id out of range");
+    for (int i = 0; i < d->num_materials; i++)
+        if (d->materials[i].type >= LJX_MAT_COUNT) fail(LJX_ERR_INVALID_ARGUMENT, "unknown material type");
+    for (int i = 0; i < d->num_media; i++) {
+        if (d->media[i].type != LJX_MEDIUM_HOMOGENEOUS)
+            fail(LJX_ERR_UNSUPPORTED, "heterogeneous media are not supported by this build");
+        if (d->media[i].phase_type > 1) fail(LJX_ERR_INVALID_ARGUMENT, "unknown phase function");
+    }
+    for (int i = 0; i < d->num_shapes; i++) {
+        const ljx_shape_desc &s = d->shapes[i];
+        if (s.type > 1) fail(LJX_ERR_INVALID_ARGUMENT, "unknown shape type");
+        if (s.material_id < -1 || s.material_id >= d->num_materials)
+            fail(LJX_ERR_INVALID_ARGUMENT, "shape " + std::to_string(i) + ": material id out of range");
+        if (s.area_light_id < -1 || s.area_light_id >= d->num_lights)
+            fail(LJX_ERR_INVALID_ARGUMENT, "shape " + std::to_string(i) + ": area light id out of range");
+        if (s.interior_medium_id < -1 || s.interior_medium_id >= d->num_media || s.exterior_medium_id < -1 ||
+            s.exterior_medium_id >= d->num_media)
+            fail(LJX_ERR_INVALID_ARGUMENT, "shape " + std::to_string(i) + ": medium id out of range");
+        if (s.type == LJX_SHAPE_TRIANGLE_MESH) {
+            if (s.num_triangles && (!s.positions || !s.indices))
+                fail(LJX_ERR_INVALID_ARGUMENT, "shape " + std::to_string(i) + ": NULL positions or indices");
+            for (uint64_t t = 0; t < 3 * s.num_triangles; t++)
+                if (s.indices[t] < 0 || uint64_t(s.indices[t]) >= s.num_vertices)
+                    fail(LJX_ERR_INVALID_ARGUMENT, "shape " + std::to_string(i) + ": vertex index out of range");
+        } else if (!(s.radius > 0)) {
+            fail(LJX_ERR_INVALID_ARGUMENT, "shape " + std::to_string(i) + ": sphere radius must be positive");
+        }
+    }
+    for (int i = 0; i < d->num_lights; i++) {
+        const ljx_light_desc &l = d->lights[i];
+        if (l.type == LJX_LIGHT_DIFFUSE_AREA) {
+            if (l.shape_id < 0 || l.shape_id >= d->num_shapes)
+                fail(LJX_ERR_INVALID_ARGUMENT, "light " + std::to_string(i) + ": shape id out of range");
+            const ljx_shape_desc &s = d->shapes[l.shape_id];
+            if (s.type == LJX_SHAPE_TRIANGLE_MESH && s.num_triangles == 0)
+                fail(LJX_ERR_INVALID_ARGUMENT, "light " + std::to_string(i) + ": emitting mesh has no triangles");
+        } else if (l.type == LJX_LIGHT_ENVMAP) {
+            if (l.env_width != 1 || l.env_height != 1 || !l.env_rgb)
+                fail(LJX_ERR_UNSUPPORTED, "only 1x1 (constant) environment maps are supported by this build");
+        } else {
+            fail(LJX_ERR_INVALID_ARGUMENT, "unknown light type");
+        }
+    }
+    if (d->envmap_light_id < -1 || d->envmap_light_id >= d->num_lights)
+        fail(LJX_ERR_INVALID_ARGUMENT, "envmap light id out of range");
+}
+
+void build_scene(Scene &sc, const ljx_scene_desc *d) {
+    validate(d);
+    auto t0 = std::chrono::steady_clock::now();
+    DeviceScene &S = sc.ds;
+    sc.options = d->options;
+    sc.num_materials = d->num_materials;
+    sc.width = d->camera.width;
+    sc.height = d->camera.height;
+    sc.has_media = d->num_media > 0;
+
+    // ---- flatten geometry
+    std::vector<float> vpos, vnrm, vuv;
+    std::vector<int> tri_vidx, prim_shape, prim_local;
+    std::vector<ShapeRec> shapes(d->num_shapes);
+    std::vector<PrimBox> boxes;
+    int ntri = 0;
+    for (int i = 0; i < d->num_shapes; i++)
+        if (d->shapes[i].type == LJX_SHAPE_TRIANGLE_MESH) ntri += int(d->shapes[i].num_triangles);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    std::vector<double> tri_cdf;
+    std::vector<char> light_shape(d->num_shapes, 0);
+    for (int i = 0; i < d->num_lights; i++)
+        if (d->lights[i].type == LJX_LIGHT_DIFFUSE_AREA) light_shape[d->lights[i].shape_id] = 1;
+    for (int i = 0; i < d->num_shapes; i++) {
+        const ljx_shape_desc &s = d->shapes[i];
+        ShapeRec &r = shapes[i];
+        std::memset(&r, 0, sizeof(r));
+        r.type = int(s.type);
+        r.material_id = s.material_id;
+        r.area_light_id = s.area_light_id;
+        r.interior_medium_id = s.interior_medium_id;
+        r.exterior_medium_id = s.exterior_medium_id;
+        r.cdf_offset = -1;
+        if (s.type == LJX_SHAPE_TRIANGLE_MESH) {
+            int voff = int(vpos.size() / 3);
+            r.has_normals = s.normals != nullptr;
+            r.has_uvs = s.uvs != nullptr;
+            r.tri_offset = int(tri_vidx.size() / 3);
+            r.num_tris = int(s.num_triangles);
+            for (uint64_t v = 0; v < s.num_vertices; v++) {
+                for (int k = 0; k < 3; k++) {
+                    float p = s.positions[3 * v + k];
+                    vpos.push_back(p);
+                    lo[k] = std::min(lo[k], p);
+                    hi[k] = std::max(hi[k], p);
+                    vnrm.push_back(s.normals ? s.normals[3 * v + k] : 0.f);
+                }
+                vuv.push_back(s.uvs ? s.uvs[2 * v] : 0.f);
+                vuv.push_back(s.uvs ? s.uvs[2 * v + 1] : 0.f);
+            }
+            // triangle areas in FP64 (src/shapes/triangle_mesh.inl:60-75)
+            std::vector<double> areas(s.num_triangles);
+            double total = 0;
+            for (uint64_t t = 0; t < s.num_triangles; t++) {
+                int a = s.indices[3 * t], b = s.indices[3 * t + 1], c = s.indices[3 * t + 2];
+                tri_vidx.push_back(a + voff);
+                tri_vidx.push_back(b + voff);
+                tri_vidx.push_back(c + voff);
+                prim_shape.push_back(i);
+                prim_local.push_back(int(t));
+                PrimBox pb;
+                double e1[3], e2[3];
+                for (int k = 0; k < 3; k++) {
+                    float pa = s.positions[3 * a + k], pbk = s.positions[3 * b + k], pc = s.positions[3 * c + k];
+                    pb.lo[k] = std::min(pa, std::min(pbk, pc));
+                    pb.hi[k] = std::max(pa, std::max(pbk, pc));
+                    e1[k] = double(pbk) - double(pa);
+                    e2[k] = double(pc) - double(pa);
+                }
+                boxes.push_back(pb);
+                double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                       cz = e1[0] * e2[1] - e1[1] * e2[0];
+                areas[t] = std::sqrt(cx * cx + cy * cy + cz * cz) / 2;
+                total += areas[t];
+            }
+            r.total_area = total;
+            if ((s.area_light_id >= 0 || light_shape[i]) && s.num_triangles > 0) {
+                std::vector<double> pmf, cdf;
+                make_dist(areas, pmf, cdf);
+                r.cdf_offset = int(tri_cdf.size());
+                tri_cdf.insert(tri_cdf.end(), cdf.begin(), cdf.end());
+            }
+        } else {
+            r.center[0] = s.center[0];
+            r.center[1] = s.center[1];
+            r.center[2] = s.center[2];
+            r.radius = s.radius;
+            r.total_area = 4 * kPi * s.radius * s.radius;
+        }
+    }
+    int nsph = 0;
+    for (int i = 0; i < d->num_shapes; i++) {
+        const ljx_shape_desc &s = d->shapes[i];
+        if (s.type != LJX_SHAPE_SPHERE) continue;
+        prim_shape.push_back(i);
+        prim_local.push_back(0);
+        PrimBox pb;
+        for (int k = 0; k < 3; k++) {
+            pb.lo[k] = float(s.center[k] - s.radius);
+            pb.hi[k] = float(s.center[k] + s.radius);
+            lo[k] = std::min(lo[k], pb.lo[k]);
+            hi[k] = std::max(hi[k], pb.hi[k]);
+        }
+        boxes.push_back(pb);
+        nsph++;
+    }
+    // ---- scene bounds (src/scene.cpp:30-34)
+    if (boxes.empty()) {
+        for (int k = 0; k < 3; k++) lo[k] = hi[k] = 0;
+    }
+    double dx = double(hi[0]) - lo[0], dy = double(hi[1]) - lo[1], dz = double(hi[2]) - lo[2];
+    sc.bound_radius = std::sqrt(dx * dx + dy * dy + dz * dz) / 2;
+    for (int k = 0; k < 3; k++) sc.bound_center[k] = (double(lo[k]) + double(hi[k])) / 2;
+    S.bound_radius = sc.bound_radius;
+    S.eps = std::min(sc.bound_radius * 1e-5, 0.01);
+
+    // ---- BVH
+    BvhBuildResult bvh = build_bvh_sah(boxes, 8, 0);
+    sc.bvh_depth = bvh.max_depth;
+    std::vector<float4> prims(bvh.slot_prim.size() * 3);
+    for (size_t slot = 0; slot < bvh.slot_prim.size(); slot++) {
+        int gid = bvh.slot_prim[slot];
+        if (gid < ntri) {
+            const int *vi = &tri_vidx[3 * gid];
+            for (int k = 0; k < 3; k++) {
+                const float *p = &vpos[3 * vi[k]];
+                float w = k == 0 ? __builtin_bit_cast(float, gid) : (k == 1 ? __builtin_bit_cast(float, kPrimTri) : 0.f);
+                prims[3 * slot + k] = make_float4(p[0], p[1], p[2], w);
+            }
+        } else {
+            const ljx_shape_desc &s = d->shapes[prim_shape[gid]];
+            prims[3 * slot] = make_float4(float(s.center[0]), float(s.center[1]), float(s.center[2]),
+                                          __builtin_bit_cast(float, gid));
+            prims[3 * slot + 1] = make_float4(float(s.radius), 0.f, 0.f, __builtin_bit_cast(float, kPrimSphere));
+            prims[3 * slot + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+
+    // ---- materials
+    std::vector<MatRec> mats(d->num_materials);
+    for (int i = 0; i < d->num_materials; i++) {
+        const ljx_material_desc &m = d->materials[i];
+        MatRec &r = mats[i];
+        std::memset(&r, 0, sizeof(r));
+        r.type = int(m.type);
+        r.a = D3{m.color_a[0], m.color_a[1], m.color_a[2]};
+        r.b = D3{m.color_b[0], m.color_b[1], m.color_b[2]};
+        r.roughness = m.roughness;
+        r.eta = m.eta;
+        r.specular_transmission = m.specular_transmission;
+        r.metallic = m.metallic;
+        r.subsurface = m.subsurface;
+        r.specular = m.specular;
+        r.specular_tint = m.specular_tint;
+        r.anisotropic = m.anisotropic;
+        r.sheen = m.sheen;
+        r.sheen_tint = m.sheen_tint;
+        r.clearcoat = m.clearcoat;
+        r.clearcoat_gloss = m.clearcoat_gloss;
+    }
+    sc.mat_types_present = 0;
+    for (int i = 0; i < d->num_shapes; i++) {
+        int mid = d->shapes[i].material_id;
+        if (mid >= 0) sc.mat_types_present |= 1u << d->materials[mid].type;
+    }
+
+    // ---- lights and their selection table (src/scene.cpp:38-52)
+    std::vector<LightRec> lights(d->num_lights);
+    std::vector<double> power(d->num_lights);
+    for (int i = 0; i < d->num_lights; i++) {
+        const ljx_light_desc &l = d->lights[i];
+        LightRec &r = lights[i];
+        std::memset(&r, 0, sizeof(r));
+        r.type = int(l.type);
+        r.shape_id = l.shape_id;
+        r.intensity = D3{l.intensity[0], l.intensity[1], l.intensity[2]};
+        if (l.type == LJX_LIGHT_ENVMAP) {
+            r.env_value = D3{l.env_rgb[0], l.env_rgb[1], l.env_rgb[2]};
+            double inv[16];
+            if (!mat_inv4(l.to_world, inv)) fail(LJX_ERR_INVALID_ARGUMENT, "envmap to_world is singular");
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) {
+                    r.to_world[a * 3 + b] = l.to_world[a * 4 + b];
+                    r.to_local[a * 3 + b] = inv[a * 4 + b];
+                }
+            r.scale = l.scale;
+            // light_power: pi r^2 * total / (w h); 1x1 texel: lum * sin(pi / 2)
+            double lum[3] = {r.env_value.x, r.env_value.y, r.env_value.z};
+            power[i] = kPi * sc.bound_radius * sc.bound_radius * (luminance_h(lum) * std::sin(kPi * 0.5));
+        } else {
+            double area = shapes[l.shape_id].total_area;
+            power[i] = luminance_h(l.intensity) * area * kPi;
+        }
+    }
+    std::vector<double> lpmf, lcdf;
+    if (d->num_lights > 0) make_dist(power, lpmf, lcdf);
+
+    // ---- media
+    std::vector<MediumRec> media(d->num_media);
+    for (int i = 0; i < d->num_media; i++) {
+        const ljx_medium_desc &m = d->media[i];
+        media[i].sigma_a = D3{m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]};
+        media[i].sigma_s = D3{m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]};
+        media[i].phase_type = int(m.phase_type);
+        media[i].g = m.g;
+    }
+
+    // ---- camera (src/camera.cpp:7-21)
+    CameraRec &cam = S.camera;
+    std::memcpy(cam.cam_to_world, d->camera.cam_to_world, sizeof(cam.cam_to_world));
+    cam.width = d->camera.width;
+    cam.height = d->camera.height;
+    cam.filter_type = int(d->camera.filter_type);
+    cam.filter_param = d->camera.filter_param;
+    cam.medium_id = d->camera.medium_id;
+    {
+        double aspect = double(cam.width) / double(cam.height);
+        double sm[16] = {-0.5, 0, 0, 0, 0, -0.5 * aspect, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+        double tm[16] = {1, 0, 0, -1.0, 0, 1, 0, -1.0 / aspect, 0, 0, 1, 0, 0, 0, 0, 1};
+        double cot = 1.0 / std::tan((kPi / 180.0) * (d->camera.fov / 2.0));
+        double pm[16] = {cot, 0, 0, 0, 0, cot, 0, 0, 0, 0, 1, -1, 0, 0, 1, 0};
+        double st[16], c2s[16];
+        mat_mul4(sm, tm, st);
+        mat_mul4(st, pm, c2s);
+        if (!mat_inv4(c2s, cam.sample_to_cam)) fail(LJX_ERR_INVALID_ARGUMENT, "camera projection is singular");
+    }
+
+    // ---- upload
+    S.nodes = sc.upload(bvh.nodes);
+    S.num_nodes = int(bvh.nodes.size());
+    S.prims = sc.upload(prims);
+    S.num_slots = int(bvh.slot_prim.size());
+    S.vpos = sc.upload(vpos);
+    S.vnrm = sc.upload(vnrm);
+    S.vuv = sc.upload(vuv);
+    S.tri_vidx = sc.upload(tri_vidx);
+    S.prim_shape = sc.upload(prim_shape);
+    S.prim_local = sc.upload(prim_local);
+    S.shapes = sc.upload(shapes);
+    S.materials = sc.upload(mats);
+    S.lights = sc.upload(lights);
+    S.media = sc.upload(media);
+    S.light_pmf = sc.upload(lpmf);
+    S.light_cdf = sc.upload(lcdf);
+    S.tri_cdf = sc.upload(tri_cdf);
+    S.num_lights = d->num_lights;
+    S.envmap_light_id = d->envmap_light_id;
+    S.num_triangles = ntri;
+    (void)nsph;
+    sc.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ------------------------------------------------------------ trace kernels
+template <bool ANY>
+__global__ __launch_bounds__(256) void k_trace_batch(DeviceScene S, const ljx_ray *rays, size_t n, ljx_hit *hits,
+                                                   uint8_t *occ, int *spill, unsigned long long *stat, int stats) {
+    __shared__ int stack[kStackLds * 256];
+    int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    int stride = gridDim.x * blockDim.x;
+    uint32_t nn = 0, np = 0;
+    for (size_t i = gtid; i < n; i += stride) {
+        ljx_ray r = rays[i];
+        TraceHit h;
+        bool ok = false;
+        if (S.num_nodes > 0) {
+            if (stats)
+                ok = trace_ray<ANY, true>(S.nodes, S.prims, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1],
+                                          r.dir[2], r.tnear, r.tfar, stack + threadIdx.x, blockDim.x,
+                                          spill + gtid, stride, h, nn, np);
+            else
+                ok = trace_ray<ANY, false>(S.nodes, S.prims, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1],
+                                           r.dir[2], r.tnear, r.tfar, stack + threadIdx.x, blockDim.x,
+                                           spill + gtid, stride, h, nn, np);
+        }
+        if (ANY) {
+            occ[i] = ok ? 1 : 0;
+        } else {
+            ljx_hit o;
+            if (!ok) {
+                o.t = 0;
+                o.u = o.v = 0;
+                o.shape_id = -1;
+                o.prim_id = -1;
+                o.ng[0] = o.ng[1] = o.ng[2] = 0;
+            } else {
+                float4 p0 = S.prims[3 * h.slot], p1 = S.prims[3 * h.slot + 1];
+                int gid = __float_as_int(p0.w);
+                o.t = h.t;
+                o.u = h.u;
+                o.v = h.v;
+                o.shape_id = S.prim_shape[gid];
+                o.prim_id = S.prim_local[gid];
+                if (__float_as_int(p1.w) == kPrimTri) {
+                    float4 p2 = S.prims[3 * h.slot + 2];
+                    float e1x = p1.x - p0.x, e1y = p1.y - p0.y, e1z = p1.z - p0.z;
+                    float e2x = p2.x - p0.x, e2y = p2.y - p0.y, e2z = p2.z - p0.z;
+                    o.ng[0] = e1y * e2z - e1z * e2y;
+                    o.ng[1] = e1z * e2x - e1x * e2z;
+                    o.ng[2] = e1x * e2y - e1y * e2x;
+                } else {
+                    o.ng[0] = r.org[0] + h.t * r.dir[0] - p0.x;
+                    o.ng[1] = r.org[1] + h.t * r.dir[1] - p0.y;
+                    o.ng[2] = r.org[2] + h.t * r.dir[2] - p0.z;
+                }
+            }
+            hits[i] = o;
+        }
+    }
+    if (stats) {
+        atomicAdd(&stat[S_NODES], (unsigned long long)nn);
+        atomicAdd(&stat[S_PRIMS], (unsigned long long)np);
+    }
+}
+
+__global__ void k_bsdf_batch(DeviceScene S, const ljx_bsdf_query *q, size_t n, ljx_bsdf_result *out) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ljx_bsdf_query a = q[i];
+    const MatRec &m = S.materials[a.material_id];
+    ShadeGeom g;
+    g.ng = D3{a.geometric_normal[0], a.geometric_normal[1], a.geometric_normal[2]};
+    g.sf = Frame{D3{a.frame_x[0], a.frame_x[1], a.frame_x[2]}, D3{a.frame_y[0], a.frame_y[1], a.frame_y[2]},
+                 D3{a.frame_n[0], a.frame_n[1], a.frame_n[2]}};
+    D3 din{a.dir_in[0], a.dir_in[1], a.dir_in[2]}, dout{a.dir_out[0], a.dir_out[1], a.dir_out[2]};
+    bool tv = a.transport_to_view != 0;
+    D3 f = bsdf_eval<-1>(m, din, dout, g, tv);
+    double pdf = bsdf_pdf<-1>(m, din, dout, g);
+    BsdfSample s = bsdf_sample<-1>(m, din, g, a.rnd_uv[0], a.rnd_uv[1], a.rnd_w);
+    ljx_bsdf_result r;
+    r.f[0] = f.x;
+    r.f[1] = f.y;
+    r.f[2] = f.z;
+    r.pdf = pdf;
+    r.sample_valid = s.valid;
+    r.sample_dir[0] = s.valid ? s.dir.x : 0;
+    r.sample_dir[1] = s.valid ? s.dir.y : 0;
+    r.sample_dir[2] = s.valid ? s.dir.z : 0;
+    r.sample_eta = s.valid ? s.eta : 0;
+    r.sample_roughness = s.valid ? s.roughness : 0;
+    out[i] = r;
+}
+
+Scene *as_scene(void *p) {
+    if (!p) fail(LJX_ERR_INVALID_ARGUMENT, "scene handle is NULL");
+    return static_cast<Scene *>(p);
+}
+
+int trace_grid_for(const Scene &sc) { return std::max(1, sc.num_cu * 4); }
+
+struct SpillBuf {
+    int *p = nullptr;
+    ~SpillBuf() {
+        if (p) hipFree(p);
+    }
+};
+
+// ------------------------------------------------------------ render loop
+struct Timer {
+    bool on;
+    hipStream_t s;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];  // extend, shadow, shade, other
+    void begin(int k, hipEvent_t &a) {
+        if (!on) return;
+        HIPX(hipEventCreate(&a));
+        HIPX(hipEventRecord(a, s));
+    }
+    void end(int k, hipEvent_t a) {
+        if (!on) return;
+        hipEvent_t b;
+        HIPX(hipEventCreate(&b));
+        HIPX(hipEventRecord(b, s));
+        ev[k].push_back({a, b});
+    }
+    double total(int k) {
+        double t = 0;
+        for (auto &e : ev[k]) {
+            float ms = 0;
+            HIPX(hipEventElapsedTime(&ms, e.first, e.second));
+            t += ms;
+            hipEventDestroy(e.first);
+            hipEventDestroy(e.second);
+        }
+        ev[k].clear();
+        return t;
+    }
+};
+
+void ensure_ctx(Scene &sc, int capacity) {
+    if (sc.rc && sc.rc->capacity >= capacity && sc.rc_alloc_W == sc.width && sc.rc_alloc_H == sc.height) return;
+    sc.release_ctx();
+    auto rc = std::make_unique<RenderCtx>();
+    rc->capacity = capacity;
+    rc->trace_grid = trace_grid_for(sc);
+    PathBufs &P = rc->P;
+    size_t C = size_t(capacity);
+    P.capacity = capacity;
+    P.rng = sc.alloc<uint64_t>(C, rc->bufs);
+    P.tp = sc.alloc<float4>(C, rc->bufs);
+    P.rad = sc.alloc<float4>(C, rc->bufs);
+    P.prev = sc.alloc<float4>(C, rc->bufs);
+    P.depth = sc.alloc<int>(C, rc->bufs);
+    P.ray_o = sc.alloc<float4>(C, rc->bufs);
+    P.ray_d = sc.alloc<float4>(C, rc->bufs);
+    P.hit = sc.alloc<float4>(C, rc->bufs);
+    P.extq0 = sc.alloc<uint32_t>(C, rc->bufs);
+    P.extq1 = sc.alloc<uint32_t>(C, rc->bufs);
+    P.bucket = sc.alloc<uint32_t>(C * kNumBuckets, rc->bufs);
+    P.shq = sc.alloc<uint32_t>(C, rc->bufs);
+    P.sh_o = sc.alloc<float4>(C, rc->bufs);
+    P.sh_d = sc.alloc<float4>(C, rc->bufs);
+    P.sh_c = sc.alloc<float4>(C, rc->bufs);
+    P.ctr = sc.alloc<int>(C_TOTAL, rc->bufs);
+    P.stat = sc.alloc<unsigned long long>(S_NUM, rc->bufs);
+    P.spill = sc.alloc<int>(size_t(rc->trace_grid) * 256 * kStackSpill, rc->bufs);
+    P.accum = sc.alloc<double>(size_t(sc.width) * sc.height * 3, rc->bufs);
+    vol_alloc(sc.has_media, C, P, rc->bufs, [&](size_t bytes) {
+        void *p = nullptr;
+        HIPX(hipMalloc(&p, bytes));
+        return p;
+    });
+    sc.rc_alloc_W = sc.width;
+    sc.rc_alloc_H = sc.height;
+    sc.rc = std::move(rc);
+}
+
+template <class K, class... A>
+void launch(K kernel, int grid, int block, hipStream_t s, A... args) {
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s, args...);
+    HIPX(hipGetLastError());
+}
+
+void shade_all(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s) {
+    const DeviceScene &S = sc.ds;
+    unsigned m = sc.mat_types_present;
+    if (m & (1u << 0)) launch(k_shade<0>, grid, 256, s, S, P, cur);
+    if (m & (1u << 1)) launch(k_shade<1>, grid, 256, s, S, P, cur);
+    if (m & (1u << 2)) launch(k_shade<2>, grid, 256, s, S, P, cur);
+    if (m & (1u << 3)) launch(k_shade<3>, grid, 256, s, S, P, cur);
+    if (m & (1u << 4)) launch(k_shade<4>, grid, 256, s, S, P, cur);
+    if (m & (1u << 5)) launch(k_shade<5>, grid, 256, s, S, P, cur);
+    if (m & (1u << 6)) launch(k_shade<6>, grid, 256, s, S, P, cur);
+    if (m & (1u << 7)) launch(k_shade<7>, grid, 256, s, S, P, cur);
+    if (m & (1u << 8)) launch(k_shade<8>, grid, 256, s, S, P, cur);
+    launch(k_miss, grid, 256, s, S, P);
+}
+
+void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options *ex, float *out_dev,
+                 ljx_render_stats *stats) {
+    auto t_start = std::chrono::steady_clock::now();
+    ljx_render_options opt = o ? *o : sc.options;
+    if (opt.samples_per_pixel <= 0) fail(LJX_ERR_INVALID_ARGUMENT, "samples_per_pixel must be positive");
+    bool vol = opt.integrator == LJX_INTEGRATOR_VOLPATH;
+    if (opt.integrator != LJX_INTEGRATOR_PATH && !vol)
+        fail(LJX_ERR_UNSUPPORTED, "only the path and volpath integrators are implemented on the device");
+    if (vol && !kVolpathEnabled) fail(LJX_ERR_UNSUPPORTED, "volpath is not enabled in this build");
+    if (vol && sc.ds.envmap_light_id >= 0)
+        fail(LJX_ERR_UNSUPPORTED, "volpath assumes no environment map (handouts/homework2.tex:180)");
+    if (opt.max_depth < -1) fail(LJX_ERR_INVALID_ARGUMENT, "max_depth must be >= -1");
+    ljx_exec_options e{};
+    if (ex) e = *ex;
+    HIPX(hipSetDevice(sc.device));
+    hipStream_t s = sc.stream;
+    int W = sc.width, H = sc.height;
+    int ntx8 = (W + 7) / 8, nty8 = (H + 7) / 8;
+    int npix = ntx8 * nty8 * 64;
+    int spp = opt.samples_per_pixel;
+    long long want = e.max_paths_in_flight > 0 ? e.max_paths_in_flight : (1 << 22);
+    want = std::max<long long>(want, 64);
+    int capacity;
+    int ns_per_pass, nq_per_pass;
+    if (npix <= want) {
+        ns_per_pass = int(std::min<long long>(spp, want / npix));
+        nq_per_pass = npix;
+    } else {
+        ns_per_pass = 1;
+        nq_per_pass = int((want / 64) * 64);
+    }
+    capacity = ns_per_pass * nq_per_pass;
+    ensure_ctx(sc, capacity);
+    RenderCtx &rc = *sc.rc;
+    PathBufs P = rc.P;
+    P.W = W;
+    P.H = H;
+    P.ntx8 = ntx8;
+    P.spp = spp;
+    P.max_depth = opt.max_depth;
+    P.rr_depth = opt.rr_depth;
+    P.seed = opt.seed;
+    P.volpath = vol ? 1 : 0;
+    P.max_null = opt.max_null_collisions;
+    const DeviceScene &S = sc.ds;
+    int shade_grid = std::max(1, std::min(capacity / 256 + 1, sc.num_cu * 16));
+    int tgrid = rc.trace_grid;
+    Timer tm{e.collect_timing != 0, s};
+    bool stats_on = e.collect_traversal != 0;
+
+    hipEvent_t ev;
+    tm.begin(3, ev);
+    launch(k_fill_f64, 1024, 256, s, P.accum, size_t(W) * H * 3);
+    launch(k_reset_counters, 1, 64, s, P.ctr, 0, C_TOTAL);
+    hipLaunchKernelGGL(k_reset_stats, dim3(1), dim3(64), 0, s, P.stat);
+    HIPX(hipGetLastError());
+    tm.end(3, ev);
+
+    uint64_t passes = 0;
+    int max_bounces = opt.max_depth >= 0 ? std::max(opt.max_depth, 1) : 1 << 20;
+    std::vector<int> host_ctr(C_TOTAL);
+    for (int s0 = 0; s0 < spp; s0 += ns_per_pass) {
+        int ns = std::min(ns_per_pass, spp - s0);
+        for (int q0 = 0; q0 < npix; q0 += nq_per_pass) {
+            int nq = std::min(nq_per_pass, npix - q0);
+            P.pass_s0 = s0;
+            P.pass_ns = ns;
+            P.pass_q0 = q0;
+            P.pass_nq = nq;
+            passes++;
+            tm.begin(3, ev);
+            launch(k_reset_counters, 1, 64, s, P.ctr, 0, C_TOTAL);
+            launch(k_raygen, shade_grid, 256, s, S, P);
+            tm.end(3, ev);
+            int cur = 0;
+            for (int b = 0; b < max_bounces; b++) {
+                if (opt.max_depth < 0 && b >= 8) {
+                    // RR-only paths: stop once the extension queue drains
+                    HIPX(hipMemcpyAsync(host_ctr.data(), P.ctr, sizeof(int) * C_TOTAL, hipMemcpyDeviceToHost, s));
+                    HIPX(hipStreamSynchronize(s));
+                    if (host_ctr[C_EXT + cur] == 0) break;
+                }
+                tm.begin(0, ev);
+                if (stats_on) launch(k_extend<true>, tgrid, 256, s, S, P, cur);
+                else launch(k_extend<false>, tgrid, 256, s, S, P, cur);
+                tm.end(0, ev);
+                tm.begin(2, ev);
+                if (vol) vol_shade_all(S, P, cur, shade_grid, s);
+                else shade_all(sc, P, cur, shade_grid, s);
+                tm.end(2, ev);
+                tm.begin(1, ev);
+                if (vol) vol_shadow(S, P, tgrid, s, stats_on);
+                else if (stats_on) launch(k_shadow<true>, tgrid, 256, s, S, P);
+                else launch(k_shadow<false>, tgrid, 256, s, S, P);
+                tm.end(1, ev);
+                cur ^= 1;
+            }
+            tm.begin(3, ev);
+            launch(k_accumulate, std::max(1, nq / 256 + 1), 256, s, P);
+            tm.end(3, ev);
+        }
+    }
+    tm.begin(3, ev);
+    size_t nout = size_t(W) * H * 3;
+    launch(k_resolve, std::max<int>(1, int(std::min<size_t>(nout / 256 + 1, 4096))), 256, s, (const double *)P.accum,
+           out_dev, nout, 1.0 / double(spp));
+    tm.end(3, ev);
+    unsigned long long st[S_NUM];
+    HIPX(hipMemcpyAsync(st, P.stat, sizeof(st), hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->camera_paths = uint64_t(W) * H * spp;
+        stats->extension_rays = st[S_EXT_RAYS];
+        stats->shadow_rays = st[S_SH_RAYS];
+        stats->passes = passes;
+        stats->extend_launches = tm.ev[0].size();
+        stats->shadow_launches = tm.ev[1].size();
+        stats->extend_ms = tm.total(0);
+        stats->shadow_ms = tm.total(1);
+        stats->shade_ms = tm.total(2);
+        stats->other_ms = tm.total(3);
+        stats->node_visits = st[S_NODES];
+        stats->tri_tests = st[S_PRIMS];
+        stats->shadow_node_visits = st[S_SH_NODES];
+        stats->shadow_tri_tests = st[S_SH_PRIMS];
+        stats->instrumented_rays = stats_on ? st[S_EXT_RAYS] + st[S_SH_RAYS] : 0;
+        stats->build_ms = sc.build_ms;
+        stats->bvh_depth = sc.bvh_depth;
+        stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    } else {
+        for (int k = 0; k < 4; k++) tm.total(k);
+    }
+}
+
+template <bool ANY>
+void trace_device(Scene &sc, const ljx_ray *rays, size_t n, void *out, const ljx_exec_options *ex, double *ms,
+                  ljx_render_stats *stats) {
+    HIPX(hipSetDevice(sc.device));
+    if (n == 0) {
+        if (ms) *ms = 0;
+        return;
+    }
+    int grid = trace_grid_for(sc);
+    SpillBuf spill;
+    HIPX(hipMalloc(&spill.p, size_t(grid) * 256 * kStackSpill * sizeof(int)));
+    unsigned long long *stat = nullptr;
+    HIPX(hipMalloc(&stat, sizeof(unsigned long long) * S_NUM));
+    struct Free {
+        void *p;
+        ~Free() { hipFree(p); }
+    } fstat{stat};
+    HIPX(hipMemsetAsync(stat, 0, sizeof(unsigned long long) * S_NUM, sc.stream));
+    int st_on = ex && ex->collect_traversal ? 1 : 0;
+    hipEvent_t a, b;
+    HIPX(hipEventCreate(&a));
+    HIPX(hipEventCreate(&b));
+    HIPX(hipEventRecord(a, sc.stream));
+    hipLaunchKernelGGL(k_trace_batch<ANY>, dim3(grid), dim3(256), 0, sc.stream, sc.ds, rays, n,
+                       ANY ? nullptr : static_cast<ljx_hit *>(out), ANY ? static_cast<uint8_t *>(out) : nullptr,
+                       spill.p, stat, st_on);
+    HIPX(hipGetLastError());
+    HIPX(hipEventRecord(b, sc.stream));
+    HIPX(hipEventSynchronize(b));
+    float el = 0;
+    HIPX(hipEventElapsedTime(&el, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (ms) *ms = el;
+    if (stats) {
+        unsigned long long h[S_NUM];
+        HIPX(hipMemcpy(h, stat, sizeof(h), hipMemcpyDeviceToHost));
+        std::memset(stats, 0, sizeof(*stats));
+        if (ANY) {
+            stats->shadow_rays = n;
+            stats->shadow_node_visits = h[S_NODES];
+            stats->shadow_tri_tests = h[S_PRIMS];
+        } else {
+            stats->extension_rays = n;
+            stats->node_visits = h[S_NODES];
+            stats->tri_tests = h[S_PRIMS];
+        }
+        stats->build_ms = sc.build_ms;
+        stats->bvh_depth = sc.bvh_depth;
+        stats->instrumented_rays = st_on ? n : 0;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ljx_abi_version(void) { return LJX_ABI_VERSION; }
+
+int ljx_last_error(char *buf, size_t cap) {
+    if (!buf || cap == 0) return LJX_ERR_INVALID_ARGUMENT;
+    std::snprintf(buf, cap, "%s", g_last_error.c_str());
+    return LJX_OK;
+}
+
+int ljx_device_count(int *count) {
+    return guarded([&]() {
+        if (!count) fail(LJX_ERR_INVALID_ARGUMENT, "count is NULL");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        int ok = 0;
+        for (int i = 0; i < n; i++) {
+            hipDeviceProp_t p;
+            if (hipGetDeviceProperties(&p, i) == hipSuccess && std::strncmp(p.gcnArchName, "gfx950", 6) == 0) ok++;
+        }
+        *count = ok;
+    });
+}
+
+int ljx_scene_create(const ljx_scene_desc *desc, int device, void **scene_out) {
+    return guarded([&]() {
+        if (!scene_out) fail(LJX_ERR_INVALID_ARGUMENT, "scene_out is NULL");
+        *scene_out = nullptr;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) fail(LJX_ERR_NO_DEVICE, "no HIP device");
+        if (device < 0 || device >= n) fail(LJX_ERR_INVALID_ARGUMENT, "device index out of range");
+        HIPX(hipSetDevice(device));
+        auto sc = std::make_unique<Scene>();
+        sc->device = device;
+        hipDeviceProp_t prop;
+        HIPX(hipGetDeviceProperties(&prop, device));
+        sc->num_cu = prop.multiProcessorCount;
+        HIPX(hipStreamCreateWithFlags(&sc->stream, hipStreamNonBlocking));
+        build_scene(*sc, desc);
+        *scene_out = sc.release();
+    });
+}
+
+int ljx_scene_destroy(void *scene) {
+    return guarded([&]() { delete static_cast<Scene *>(scene); });
+}
+
+int ljx_scene_bounds(void *scene, double *cr) {
+    return guarded([&]() {
+        Scene *sc = as_scene(scene);
+        if (!cr) fail(LJX_ERR_INVALID_ARGUMENT, "output is NULL");
+        cr[0] = sc->bound_center[0];
+        cr[1] = sc->bound_center[1];
+        cr[2] = sc->bound_center[2];
+        cr[3] = sc->bound_radius;
+    });
+}
+
+int ljx_render_device(void *scene, const ljx_render_options *opts, const ljx_exec_options *exec, float *rgb_out_device,
+                      ljx_render_stats *stats) {
+    return guarded([&]() {
+        Scene *sc = as_scene(scene);
+        if (!rgb_out_device) fail(LJX_ERR_INVALID_ARGUMENT, "output is NULL");
+        render_impl(*sc, opts, exec, rgb_out_device, stats);
+    });
+}
+
+int ljx_render(void *scene, const ljx_render_options *opts, const ljx_exec_options *exec, float *rgb_out,
+               ljx_render_stats *stats) {
+    return guarded([&]() {
+        Scene *sc = as_scene(scene);
+        if (!rgb_out) fail(LJX_ERR_INVALID_ARGUMENT, "output is NULL");
+        HIPX(hipSetDevice(sc->device));
+        size_t bytes = size_t(sc->width) * sc->height * 3 * sizeof(float);
+        float *d = nullptr;
+        HIPX(hipMalloc(&d, bytes));
+        struct Free {
+            void *p;
+            ~Free() { hipFree(p); }
+        } f{d};
+        render_impl(*sc, opts, exec, d, stats);
+        HIPX(hipMemcpy(rgb_out, d, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int ljx_trace_closest_device(void *scene, const ljx_ray *rays, size_t n, ljx_hit *hits, const ljx_exec_options *exec,
+                             double *elapsed_ms, ljx_render_stats *stats) {
+    return guarded([&]() {
+        Scene *sc = as_scene(scene);
+        if (n && (!rays || !hits)) fail(LJX_ERR_INVALID_ARGUMENT, "NULL buffer");
+        trace_device<false>(*sc, rays, n, hits, exec, elapsed_ms, stats);
+    });
+}
+
+int ljx_trace_any_device(void *scene, const ljx_ray *rays, size_t n, uint8_t *occ, const ljx_exec_options *exec,
+                         double *elapsed_ms, ljx_render_stats *stats) {
+    return guarded([&]() {
+        Scene *sc = as_scene(scene);
+        if (n && (!rays || !occ)) fail(LJX_ERR_INVALID_ARGUMENT, "NULL buffer");
+        trace_device<true>(*sc, rays, n, occ, exec, elapsed_ms, stats);
+    });
+}
+
+int ljx_trace_closest(void *scene, const ljx_ray *rays, size_t n, ljx_hit *hits) {
+    return guarded([&]() {
+        Scene *sc = as_scene(scene);
+        if (n == 0) return;
+        if (!rays || !hits) fail(LJX_ERR_INVALID_ARGUMENT, "NULL buffer");
+        HIPX(hipSetDevice(sc->device));
+        void *dr = nullptr, *dh = nullptr;
+        HIPX(hipMalloc(&dr, n * sizeof(ljx_ray)));
+        struct Free {
+            void *p;
+            ~Free() {
+                if (p) hipFree(p);
+            }
+        } f1{dr};
+        HIPX(hipMalloc(&dh, n * sizeof(ljx_hit)));
+        Free f2{dh};
+        HIPX(hipMemcpy(dr, rays, n * sizeof(ljx_ray), hipMemcpyHostToDevice));
+        trace_device<false>(*sc, static_cast<ljx_ray *>(dr), n, dh, nullptr, nullptr, nullptr);
+        HIPX(hipMemcpy(hits, dh, n * sizeof(ljx_hit), hipMemcpyDeviceToHost));
+    });
+}
+
+int ljx_trace_any(void *scene, const ljx_ray *rays, size_t n, uint8_t *occluded) {
+    return guarded([&]() {
+        Scene *sc = as_scene(scene);
+        if (n == 0) return;
+        if (!rays || !occluded) fail(LJX_ERR_INVALID_ARGUMENT, "NULL buffer");
+        HIPX(hipSetDevice(sc->device));
+        void *dr = nullptr, *dh = nullptr;
+        HIPX(hipMalloc(&dr, n * sizeof(ljx_ray)));
+        struct Free {
+            void *p;
+            ~Free() {
+                if (p) hipFree(p);
+            }
+        } f1{dr};
+        HIPX(hipMalloc(&dh, n));
+        Free f2{dh};
+        HIPX(hipMemcpy(dr, rays, n * sizeof(ljx_ray), hipMemcpyHostToDevice));
+        trace_device<true>(*sc, static_cast<ljx_ray *>(dr), n, dh, nullptr, nullptr, nullptr);
+        HIPX(hipMemcpy(occluded, dh, n, hipMemcpyDeviceToHost));
+    });
+}
+
+int ljx_bsdf_batch(void *scene, const ljx_bsdf_query *queries, size_t n, ljx_bsdf_result *results) {
+    return guarded([&]() {
+        Scene *sc = as_scene(scene);
+        if (n == 0) return;
+        if (!queries || !results) fail(LJX_ERR_INVALID_ARGUMENT, "NULL buffer");
+        HIPX(hipSetDevice(sc->device));
+        for (size_t i = 0; i < n; i++)
+            if (queries[i].material_id < 0 || queries[i].material_id >= sc->num_materials)
+                fail(LJX_ERR_INVALID_ARGUMENT, "material id out of range");
+        void *dq = nullptr, *dr = nullptr;
+        HIPX(hipMalloc(&dq, n * sizeof(ljx_bsdf_query)));
+        struct Free {
+            void *p;
+            ~Free() {
+                if (p) hipFree(p);
+            }
+        } f1{dq};
+        HIPX(hipMalloc(&dr, n * sizeof(ljx_bsdf_result)));
+        Free f2{dr};
+        HIPX(hipMemcpy(dq, queries, n * sizeof(ljx_bsdf_query), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_bsdf_batch, dim3(unsigned((n + 127) / 128)), dim3(128), 0, sc->stream, sc->ds,
+                           static_cast<const ljx_bsdf_query *>(dq), n, static_cast<ljx_bsdf_result *>(dr));
+        HIPX(hipGetLastError());
+        HIPX(hipStreamSynchronize(sc->stream));
+        HIPX(hipMemcpy(results, dr, n * sizeof(ljx_bsdf_result), hipMemcpyDeviceToHost));
+    });
+}
+
+int ljx_device_alloc(int device, size_t bytes, void **ptr) {
+    return guarded([&]() {
+        if (!ptr) fail(LJX_ERR_INVALID_ARGUMENT, "ptr is NULL");
+        HIPX(hipSetDevice(device));
+        HIPX(hipMalloc(ptr, bytes));
+    });
+}
+
+int ljx_device_free(void *ptr) {
+    return guarded([&]() { HIPX(hipFree(ptr)); });
+}
+
+int ljx_copy_to_device(void *dst, const void *src, size_t bytes) {
+    return guarded([&]() { HIPX(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+
+int ljx_copy_to_host(void *dst, const void *src, size_t bytes) {
+    return guarded([&]() { HIPX(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+
+}  // extern "C"

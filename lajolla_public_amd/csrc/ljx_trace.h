@@ -1,0 +1,233 @@
+// BVH traversal for gfx950: closest hit and any hit, FP32, one ray per lane.
+//
+// Replaces Embree's rtcIntersect1 / rtcOccluded1 as lajolla calls them
+// (src/intersection.cpp:7-85).  Node layout: two child boxes per 64-byte node
+// (ljx_device.h BvhNode), loaded as four 16-byte vector loads.  The
+// traversal stack lives in LDS (STACK_LDS entries per lane, lane-interleaved
+// so a wave's pushes hit 64 consecutive banks) and spills to a per-lane
+// global area beyond that; the builder bounds the tree depth so the spill
+// area never overflows.  Triangles use the watertight test of Woop, Benthin
+// and Wald (JCGT 2013) with an FP64 re-test of edge hits.
+#pragma once
+
+#include "ljx_device.h"
+
+namespace ljx {
+
+constexpr int kStackLds = 24;     // per-lane LDS stack entries
+constexpr int kStackSpill = 96;   // per-lane spill entries (tree depth <= kStackLds + kStackSpill)
+constexpr int kMaxTreeDepth = kStackLds + kStackSpill - 2;
+constexpr int kPrimTri = 0, kPrimSphere = 1;
+
+struct TraceHit {
+    float t, u, v;
+    int slot;
+};
+
+struct RayPre {
+    float ox, oy, oz;
+    float idx, idy, idz;
+    float oidx, oidy, oidz;  // o * (1/d)
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ float sel3(float a, float b, float c, int k) { return k == 0 ? a : (k == 1 ? b : c); }
+
+__device__ __forceinline__ RayPre ray_setup(float ox, float oy, float oz, float dx, float dy, float dz) {
+    RayPre r;
+    r.ox = ox;
+    r.oy = oy;
+    r.oz = oz;
+    // keep 1/d finite so that 0 * inf never arises in the slab test
+    const float tiny = 1e-20f;
+    float sdx = fabsf(dx) < tiny ? copysignf(tiny, dx) : dx;
+    float sdy = fabsf(dy) < tiny ? copysignf(tiny, dy) : dy;
+    float sdz = fabsf(dz) < tiny ? copysignf(tiny, dz) : dz;
+    r.idx = 1.0f / sdx;
+    r.idy = 1.0f / sdy;
+    r.idz = 1.0f / sdz;
+    r.oidx = ox * r.idx;
+    r.oidy = oy * r.idy;
+    r.oidz = oz * r.idz;
+    float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float dkz = sel3(dx, dy, dz, kz);
+    if (dkz < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    r.Sx = sel3(dx, dy, dz, kx) / dkz;
+    r.Sy = sel3(dx, dy, dz, ky) / dkz;
+    r.Sz = 1.0f / dkz;
+    return r;
+}
+
+// Slab test of one child box; returns entry distance, or +inf on a miss.
+__device__ __forceinline__ float box_entry(const RayPre &r, float lx, float hx, float ly, float hy, float lz, float hz,
+                                           float tmin, float tmax) {
+    float tx0 = __builtin_fmaf(lx, r.idx, -r.oidx), tx1 = __builtin_fmaf(hx, r.idx, -r.oidx);
+    float ty0 = __builtin_fmaf(ly, r.idy, -r.oidy), ty1 = __builtin_fmaf(hy, r.idy, -r.oidy);
+    float tz0 = __builtin_fmaf(lz, r.idz, -r.oidz), tz1 = __builtin_fmaf(hz, r.idz, -r.oidz);
+    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+    // conservative widening (Ize 2013): 1 + 2 * gamma(3) in FP32
+    tf *= 1.00000036f;
+    return tn <= tf ? tn : __builtin_huge_valf();
+}
+
+// Watertight ray/triangle test.  On a hit inside (tmin, tmax] writes t and the
+// barycentrics of v1 (u) and v2 (v).
+__device__ __forceinline__ bool tri_test(const RayPre &r, float4 a4, float4 b4, float4 c4, float tmin, float tmax,
+                                         float &t_out, float &u_out, float &v_out) {
+    float Ax0 = a4.x - r.ox, Ay0 = a4.y - r.oy, Az0 = a4.z - r.oz;
+    float Bx0 = b4.x - r.ox, By0 = b4.y - r.oy, Bz0 = b4.z - r.oz;
+    float Cx0 = c4.x - r.ox, Cy0 = c4.y - r.oy, Cz0 = c4.z - r.oz;
+    float Akz = sel3(Ax0, Ay0, Az0, r.kz), Bkz = sel3(Bx0, By0, Bz0, r.kz), Ckz = sel3(Cx0, Cy0, Cz0, r.kz);
+    float Ax = sel3(Ax0, Ay0, Az0, r.kx) - r.Sx * Akz;
+    float Ay = sel3(Ax0, Ay0, Az0, r.ky) - r.Sy * Akz;
+    float Bx = sel3(Bx0, By0, Bz0, r.kx) - r.Sx * Bkz;
+    float By = sel3(Bx0, By0, Bz0, r.ky) - r.Sy * Bkz;
+    float Cx = sel3(Cx0, Cy0, Cz0, r.kx) - r.Sx * Ckz;
+    float Cy = sel3(Cx0, Cy0, Cz0, r.ky) - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        // edge case: redo the 2D edge functions in FP64 for watertightness
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    float rcp = 1.0f / det;
+    float t = T * rcp;
+    if (!(t >= tmin && t <= tmax)) return false;
+    t_out = t;
+    u_out = V * rcp;
+    v_out = W * rcp;
+    return true;
+}
+
+// Sphere user primitive (src/shapes/sphere.inl:12-106): stable quadratic in
+// FP64, nearest root in [tmin, tmax).  u, v carry azimuth/2pi, elevation/pi.
+__device__ __forceinline__ bool sphere_test(const RayPre &r, float dx, float dy, float dz, float4 c4, float rad,
+                                            float tmin, float tmax, float &t_out, float &u_out, float &v_out) {
+    double vx = double(r.ox) - c4.x, vy = double(r.oy) - c4.y, vz = double(r.oz) - c4.z;
+    double A = double(dx) * dx + double(dy) * dy + double(dz) * dz;
+    double B = 2 * (double(dx) * vx + double(dy) * vy + double(dz) * vz);
+    double Cc = vx * vx + vy * vy + vz * vz - double(rad) * rad;
+    double t0, t1;
+    if (A == 0) {
+        if (B == 0) return false;
+        t0 = t1 = -Cc / B;
+    } else {
+        double disc = B * B - 4 * A * Cc;
+        if (disc < 0) return false;
+        double rd = sqrt(disc);
+        if (B >= 0) {
+            t0 = (-B - rd) / (2 * A);
+            t1 = 2 * Cc / (-B - rd);
+        } else {
+            t0 = 2 * Cc / (-B + rd);
+            t1 = (-B + rd) / (2 * A);
+        }
+    }
+    if (t0 > t1) {
+        double tt = t0;
+        t0 = t1;
+        t1 = tt;
+    }
+    double t = -1;
+    if (t0 >= tmin && t0 < tmax) t = t0;
+    if (t1 >= tmin && t1 < tmax && t < 0) t = t1;
+    if (!(t >= tmin && t < tmax)) return false;
+    double px = r.ox + t * dx - c4.x, py = r.oy + t * dy - c4.y, pz = r.oz + t * dz - c4.z;
+    double el = acos(clampd(py / rad, -1.0, 1.0));
+    double az = atan2(pz / rad, px / rad);
+    t_out = float(t);
+    u_out = float(az * kInvTwoPi);
+    v_out = float(el * kInvPi);
+    return true;
+}
+
+// Trace one ray.  stack: this lane's LDS column base (entries at stack[k*stride]);
+// spill: this lane's global spill column.  Returns true on a hit.
+template <bool ANY, bool STATS>
+__device__ __forceinline__ bool trace_ray(const BvhNode *__restrict__ nodes, const float4 *__restrict__ prims,
+                                          float ox, float oy, float oz, float dx, float dy, float dz, float tmin,
+                                          float tmax, int *stack, int stride, int *spill, int spill_stride,
+                                          TraceHit &hit, uint32_t &n_nodes, uint32_t &n_prims) {
+    RayPre r = ray_setup(ox, oy, oz, dx, dy, dz);
+    int sp = 0;
+    int node = 0;  // root is always an inner node
+    bool found = false;
+    hit.slot = -1;
+    while (true) {
+        if (node >= 0) {
+            const BvhNode *nd = nodes + node;
+            float4 a = nd->lxy, b = nd->rxy, z = nd->zz;
+            int4 ch = nd->ch;
+            if (STATS) n_nodes++;
+            float e0 = box_entry(r, a.x, a.y, a.z, a.w, z.x, z.y, tmin, tmax);
+            float e1 = box_entry(r, b.x, b.y, b.z, b.w, z.z, z.w, tmin, tmax);
+            bool h0 = e0 != __builtin_huge_valf(), h1 = e1 != __builtin_huge_valf();
+            if (h0 && h1) {
+                int nearc = ch.x, farc = ch.y;
+                if (e1 < e0) {
+                    nearc = ch.y;
+                    farc = ch.x;
+                }
+                if (sp < kStackLds) stack[sp * stride] = farc;
+                else spill[(sp - kStackLds) * spill_stride] = farc;
+                sp++;
+                node = nearc;
+                continue;
+            } else if (h0 || h1) {
+                node = h0 ? ch.x : ch.y;
+                continue;
+            }
+        } else {
+            int code = ~node;
+            int first = code >> 3, cnt = (code & 7) + 1;
+            for (int k = 0; k < cnt; k++) {
+                int slot = first + k;
+                float4 p0 = prims[3 * slot], p1 = prims[3 * slot + 1];
+                float t, u, v;
+                bool ok;
+                if (STATS) n_prims++;
+                if (__float_as_int(p1.w) == kPrimTri) {
+                    float4 p2 = prims[3 * slot + 2];
+                    ok = tri_test(r, p0, p1, p2, tmin, tmax, t, u, v);
+                } else {
+                    ok = sphere_test(r, dx, dy, dz, p0, p1.x, tmin, tmax, t, u, v);
+                }
+                if (ok) {
+                    found = true;
+                    hit.t = t;
+                    hit.u = u;
+                    hit.v = v;
+                    hit.slot = slot;
+                    if (ANY) return true;
+                    tmax = t;
+                }
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        node = sp < kStackLds ? stack[sp * stride] : spill[(sp - kStackLds) * spill_stride];
+    }
+    return found;
+}
+
+}  // namespace ljx
