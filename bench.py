@@ -31,11 +31,25 @@ def main():
     ap.add_argument("--cpu-spp", type=int, default=0, help="spp of the bounded CPU baseline sample (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-paths", type=int, default=0)
+    ap.add_argument("--mode", choices=["render", "copycal"], default="render",
+                    help="copycal: one plain FP32 elementwise kernel over 1 GiB (read) + 1 GiB (write), "
+                         "to scale the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); every box here has one GPU.",
               file=sys.stderr)
         sys.exit(2)
+    if args.mode == "copycal":
+        import torch
+        n = (1 << 30) // 4
+        a = torch.ones(n, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            torch.mul(a, 2.0, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copycal_bytes_read": n * 4, "copycal_bytes_written": n * 4, "launches": 3}))
+        return
 
     import numpy as np
     from lajolla_public_amd import _capi as capi
@@ -148,8 +162,8 @@ def cpu_baseline(desc, cpu_spp):
     build_s = time.time() - t0
     spp = cpu_spp
     if spp <= 0:
-        # ~0.35 Msamples/s per 8 threads on this scene: aim for ~15 s
-        spp = max(1, min(desc.options["samples_per_pixel"], int(round(15 * 0.045e6 * threads /
+        # ~0.25 Msamples/s per host thread on this scene: aim for ~15 s
+        spp = max(1, min(desc.options["samples_per_pixel"], int(round(15 * 0.25e6 * threads /
                                                                           (desc.camera["width"] * desc.camera["height"])))))
     o = dict(desc.options)
     o["samples_per_pixel"] = spp

@@ -175,6 +175,7 @@ struct DeviceScene {
     // traversal
     const BvhNode *nodes;
     const float4 *prims;      // 3 float4 per leaf slot: v0|gid, v1|kind, v2|-
+    const double4 *spheres;   // centre, radius per sphere primitive
     int num_nodes;
     int num_slots;
     // geometry for shading / light sampling

@@ -87,9 +87,12 @@ __global__ void k_fill_f64(double *p, size_t n) {
     for (; i < n; i += stride) p[i] = 0.0;
 }
 
-__global__ void k_raygen(DeviceScene S, PathBufs P) {
+// compact = 0: every slot holds a valid pixel (W and H multiples of 8), so the
+// extension queue is the identity and needs no atomics.
+__global__ void k_raygen(DeviceScene S, PathBufs P, int compact) {
     int n = P.pass_ns * P.pass_nq;
     int stride = gridDim.x * blockDim.x;
+    if (!compact && blockIdx.x == 0 && threadIdx.x == 0) P.ctr[C_EXT + 0] = n;
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         int i = base + threadIdx.x;
         int s = 0, x = 0, y = 0;
@@ -110,13 +113,17 @@ __global__ void k_raygen(DeviceScene S, PathBufs P) {
             P.depth[i] = 0;
         }
         if (i < n) P.rad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        int k = wave_append(valid, &P.ctr[C_EXT + 0]);
-        if (k >= 0) P.extq0[k] = uint32_t(i);
+        if (compact) {
+            int k = wave_append(valid, &P.ctr[C_EXT + 0]);
+            if (k >= 0) P.extq0[k] = uint32_t(i);
+        } else if (i < n) {
+            P.extq0[i] = uint32_t(i);
+        }
     }
 }
 
 // Closest-hit traversal over the extension queue, then material bucketing.
-template <bool STATS>
+template <bool STATS, bool SPH>
 __global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int cur) {
     __shared__ int stack[kStackLds * 256];
     const uint32_t *q = cur ? P.extq1 : P.extq0;
@@ -139,7 +146,7 @@ __global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int c
             float4 o = P.ray_o[pid], d = P.ray_d[pid];
             TraceHit h;
             bool hit = S.num_nodes > 0 &&
-                       trace_ray<false, STATS>(S.nodes, S.prims, o.x, o.y, o.z, d.x, d.y, d.z, o.w, d.w,
+                       trace_ray<false, STATS, SPH>(TraceScene{S.nodes, S.prims, S.spheres}, o.x, o.y, o.z, d.x, d.y, d.z, o.w, d.w,
                                                stack + threadIdx.x, blockDim.x, spill, stride, h, nn, np);
             if (hit) {
                 P.hit[pid] = make_float4(h.t, h.u, h.v, __int_as_float(h.slot));
@@ -169,7 +176,7 @@ __global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int c
 }
 
 // Pending NEE contribution and the next ray of one path vertex.
-template <int MT>
+template <int MT, bool SPH>
 __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBufs &P, uint32_t pid, bool &push_ext,
                                               bool &push_sh, float4 &sh_o, float4 &sh_d, float4 &sh_c) {
     float4 h = P.hit[pid];
@@ -190,7 +197,7 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
     double eta_scale = tp4.w;
     float4 rad4 = P.rad[pid];
     D3 rad{rad4.x, rad4.y, rad4.z};
-    Vertex v = make_vertex(S, __float_as_int(h.w), h.x, h.y, h.z, org, dir);
+    Vertex v = make_vertex<SPH>(S, __float_as_int(h.w), h.x, h.y, h.z, org, dir);
     const MatRec &mat = S.materials[v.material_id];
     bool alive = true;
     push_ext = false;
@@ -289,7 +296,7 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
     P.rad[pid] = make_float4(float(rad.x), float(rad.y), float(rad.z), 0.f);
 }
 
-template <int MT>
+template <int MT, bool SPH>
 __global__ __launch_bounds__(256) void k_shade(DeviceScene S, PathBufs P, int cur) {
     int n = P.ctr[C_BUCKET + MT];
     const uint32_t *q = P.bucket + size_t(MT) * P.capacity;
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene S, PathBufs P, int cu
         uint32_t pid = 0;
         if (i < n) {
             pid = q[i];
-            shade_surface<MT>(S, P, pid, pe, ps, so, sd, sc);
+            shade_surface<MT, SPH>(S, P, pid, pe, ps, so, sd, sc);
         }
         int k = wave_append(pe, &P.ctr[C_EXT + (cur ^ 1)]);
         if (k >= 0) next[k] = pid;
@@ -349,7 +356,7 @@ __global__ __launch_bounds__(256) void k_miss(DeviceScene S, PathBufs P) {
     }
 }
 
-template <bool STATS>
+template <bool STATS, bool SPH>
 __global__ __launch_bounds__(256) void k_shadow(DeviceScene S, PathBufs P) {
     __shared__ int stack[kStackLds * 256];
     int n = P.ctr[C_SH];
@@ -365,7 +372,7 @@ __global__ __launch_bounds__(256) void k_shadow(DeviceScene S, PathBufs P) {
         float4 o = P.sh_o[i], d = P.sh_d[i];
         TraceHit h;
         bool occ = S.num_nodes > 0 &&
-                   trace_ray<true, STATS>(S.nodes, S.prims, o.x, o.y, o.z, d.x, d.y, d.z, o.w, d.w,
+                   trace_ray<true, STATS, SPH>(TraceScene{S.nodes, S.prims, S.spheres}, o.x, o.y, o.z, d.x, d.y, d.z, o.w, d.w,
                                           stack + threadIdx.x, blockDim.x, spill, stride, h, nn, np);
         if (!occ) {
             float4 c = P.sh_c[i];

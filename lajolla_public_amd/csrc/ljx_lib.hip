@@ -137,6 +137,7 @@ struct Scene {
     int width = 0, height = 0;
     int num_cu = 0;
     int num_materials = 0;
+    bool has_spheres = false;
     unsigned mat_types_present = 0;
     bool has_media = false;
     double bound_center[3] = {0, 0, 0};
@@ -328,9 +329,11 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
         }
     }
     int nsph = 0;
+    std::vector<double4> spheres;
     for (int i = 0; i < d->num_shapes; i++) {
         const ljx_shape_desc &s = d->shapes[i];
         if (s.type != LJX_SHAPE_SPHERE) continue;
+        spheres.push_back(make_double4(s.center[0], s.center[1], s.center[2], s.radius));
         prim_shape.push_back(i);
         prim_local.push_back(0);
         PrimBox pb;
@@ -368,9 +371,11 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
             }
         } else {
             const ljx_shape_desc &s = d->shapes[prim_shape[gid]];
+            int si = gid - ntri;
             prims[3 * slot] = make_float4(float(s.center[0]), float(s.center[1]), float(s.center[2]),
                                           __builtin_bit_cast(float, gid));
-            prims[3 * slot + 1] = make_float4(float(s.radius), 0.f, 0.f, __builtin_bit_cast(float, kPrimSphere));
+            prims[3 * slot + 1] = make_float4(__builtin_bit_cast(float, si), 0.f, 0.f,
+                                              __builtin_bit_cast(float, kPrimSphere));
             prims[3 * slot + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
@@ -468,6 +473,7 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
     S.nodes = sc.upload(bvh.nodes);
     S.num_nodes = int(bvh.nodes.size());
     S.prims = sc.upload(prims);
+    S.spheres = sc.upload(spheres);
     S.num_slots = int(bvh.slot_prim.size());
     S.vpos = sc.upload(vpos);
     S.vnrm = sc.upload(vnrm);
@@ -485,7 +491,7 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
     S.num_lights = d->num_lights;
     S.envmap_light_id = d->envmap_light_id;
     S.num_triangles = ntri;
-    (void)nsph;
+    sc.has_spheres = nsph > 0;
     sc.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -503,11 +509,11 @@ __global__ __launch_bounds__(256) void k_trace_batch(DeviceScene S, const ljx_ra
         bool ok = false;
         if (S.num_nodes > 0) {
             if (stats)
-                ok = trace_ray<ANY, true>(S.nodes, S.prims, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1],
+                ok = trace_ray<ANY, true, true>(TraceScene{S.nodes, S.prims, S.spheres}, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1],
                                           r.dir[2], r.tnear, r.tfar, stack + threadIdx.x, blockDim.x,
                                           spill + gtid, stride, h, nn, np);
             else
-                ok = trace_ray<ANY, false>(S.nodes, S.prims, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1],
+                ok = trace_ray<ANY, false, true>(TraceScene{S.nodes, S.prims, S.spheres}, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1],
                                            r.dir[2], r.tnear, r.tfar, stack + threadIdx.x, blockDim.x,
                                            spill + gtid, stride, h, nn, np);
         }
@@ -537,9 +543,16 @@ __global__ __launch_bounds__(256) void k_trace_batch(DeviceScene S, const ljx_ra
                     o.ng[1] = e1z * e2x - e1x * e2z;
                     o.ng[2] = e1x * e2y - e1y * e2x;
                 } else {
-                    o.ng[0] = r.org[0] + h.t * r.dir[0] - p0.x;
-                    o.ng[1] = r.org[1] + h.t * r.dir[1] - p0.y;
-                    o.ng[2] = r.org[2] + h.t * r.dir[2] - p0.z;
+                    double4 c = S.spheres[__float_as_int(p1.x)];
+                    double td, su, sv;
+                    sphere_hit_uv(c, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1], r.dir[2], r.tnear, h.t, td,
+                                  su, sv);
+                    o.t = float(td);
+                    o.u = float(su);
+                    o.v = float(sv);
+                    o.ng[0] = float(r.org[0] + td * r.dir[0] - c.x);
+                    o.ng[1] = float(r.org[1] + td * r.dir[1] - c.y);
+                    o.ng[2] = float(r.org[2] + td * r.dir[2] - c.z);
                 }
             }
             hits[i] = o;
@@ -668,19 +681,38 @@ void launch(K kernel, int grid, int block, hipStream_t s, A... args) {
     HIPX(hipGetLastError());
 }
 
-void shade_all(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s) {
+template <bool SPH>
+void shade_all_t(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s) {
     const DeviceScene &S = sc.ds;
     unsigned m = sc.mat_types_present;
-    if (m & (1u << 0)) launch(k_shade<0>, grid, 256, s, S, P, cur);
-    if (m & (1u << 1)) launch(k_shade<1>, grid, 256, s, S, P, cur);
-    if (m & (1u << 2)) launch(k_shade<2>, grid, 256, s, S, P, cur);
-    if (m & (1u << 3)) launch(k_shade<3>, grid, 256, s, S, P, cur);
-    if (m & (1u << 4)) launch(k_shade<4>, grid, 256, s, S, P, cur);
-    if (m & (1u << 5)) launch(k_shade<5>, grid, 256, s, S, P, cur);
-    if (m & (1u << 6)) launch(k_shade<6>, grid, 256, s, S, P, cur);
-    if (m & (1u << 7)) launch(k_shade<7>, grid, 256, s, S, P, cur);
-    if (m & (1u << 8)) launch(k_shade<8>, grid, 256, s, S, P, cur);
+    if (m & (1u << 0)) launch(k_shade<0, SPH>, grid, 256, s, S, P, cur);
+    if (m & (1u << 1)) launch(k_shade<1, SPH>, grid, 256, s, S, P, cur);
+    if (m & (1u << 2)) launch(k_shade<2, SPH>, grid, 256, s, S, P, cur);
+    if (m & (1u << 3)) launch(k_shade<3, SPH>, grid, 256, s, S, P, cur);
+    if (m & (1u << 4)) launch(k_shade<4, SPH>, grid, 256, s, S, P, cur);
+    if (m & (1u << 5)) launch(k_shade<5, SPH>, grid, 256, s, S, P, cur);
+    if (m & (1u << 6)) launch(k_shade<6, SPH>, grid, 256, s, S, P, cur);
+    if (m & (1u << 7)) launch(k_shade<7, SPH>, grid, 256, s, S, P, cur);
+    if (m & (1u << 8)) launch(k_shade<8, SPH>, grid, 256, s, S, P, cur);
     launch(k_miss, grid, 256, s, S, P);
+}
+void shade_all(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s) {
+    if (sc.has_spheres) shade_all_t<true>(sc, P, cur, grid, s);
+    else shade_all_t<false>(sc, P, cur, grid, s);
+}
+template <bool STATS, bool SPH>
+void trace_stage_t(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s, bool shadow) {
+    if (shadow) launch(k_shadow<STATS, SPH>, grid, 256, s, sc.ds, P);
+    else launch(k_extend<STATS, SPH>, grid, 256, s, sc.ds, P, cur);
+}
+void trace_stage(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s, bool stats, bool shadow) {
+    if (stats) {
+        if (sc.has_spheres) trace_stage_t<true, true>(sc, P, cur, grid, s, shadow);
+        else trace_stage_t<true, false>(sc, P, cur, grid, s, shadow);
+    } else {
+        if (sc.has_spheres) trace_stage_t<false, true>(sc, P, cur, grid, s, shadow);
+        else trace_stage_t<false, false>(sc, P, cur, grid, s, shadow);
+    }
 }
 
 void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options *ex, float *out_dev,
@@ -755,7 +787,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
             passes++;
             tm.begin(3, ev);
             launch(k_reset_counters, 1, 64, s, P.ctr, 0, C_TOTAL);
-            launch(k_raygen, shade_grid, 256, s, S, P);
+            launch(k_raygen, shade_grid, 256, s, S, P, (W % 8 || H % 8) ? 1 : 0);
             tm.end(3, ev);
             int cur = 0;
             for (int b = 0; b < max_bounces; b++) {
@@ -766,8 +798,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
                     if (host_ctr[C_EXT + cur] == 0) break;
                 }
                 tm.begin(0, ev);
-                if (stats_on) launch(k_extend<true>, tgrid, 256, s, S, P, cur);
-                else launch(k_extend<false>, tgrid, 256, s, S, P, cur);
+                trace_stage(sc, P, cur, tgrid, s, stats_on, false);
                 tm.end(0, ev);
                 tm.begin(2, ev);
                 if (vol) vol_shade_all(S, P, cur, shade_grid, s);
@@ -775,8 +806,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
                 tm.end(2, ev);
                 tm.begin(1, ev);
                 if (vol) vol_shadow(S, P, tgrid, s, stats_on);
-                else if (stats_on) launch(k_shadow<true>, tgrid, 256, s, S, P);
-                else launch(k_shadow<false>, tgrid, 256, s, S, P);
+                else trace_stage(sc, P, cur, tgrid, s, stats_on, true);
                 tm.end(1, ev);
                 cur ^= 1;
             }

@@ -40,6 +40,7 @@ struct Vertex {
 // compute_shading_info, src/shapes/triangle_mesh.inl:77-169 and
 // src/shapes/sphere.inl:243-268).  Ray differentials are not carried: every
 // texture here is constant, so they cannot change a value.
+template <bool SPH>
 __device__ __forceinline__ Vertex make_vertex(const DeviceScene &S, int slot, float t, float u, float v, D3 org,
                                               D3 dir) {
     Vertex vx;
@@ -55,7 +56,7 @@ __device__ __forceinline__ Vertex make_vertex(const DeviceScene &S, int slot, fl
     vx.pos = org + dir * double(t);
     D3 ng;
     Frame sf;
-    if (__float_as_int(p1.w) == kPrimTri) {
+    if (!SPH || __float_as_int(p1.w) == kPrimTri) {
         float4 p2 = S.prims[3 * slot + 2];
         D3 a = f3d(p0), b = f3d(p1), c = f3d(p2);
         ng = normalize(cross(b - a, c - a));
@@ -96,9 +97,14 @@ __device__ __forceinline__ Vertex make_vertex(const DeviceScene &S, int slot, fl
         sf = Frame{tg, normalize(cross(n, tg)), n};
     } else {
         D3 c{sh.center[0], sh.center[1], sh.center[2]};
-        ng = normalize(vx.pos - c);
         double r = sh.radius;
-        double su = double(u), sv = double(v);
+        double td, su, sv;
+        sphere_hit_uv(make_double4(c.x, c.y, c.z, r), org.x, org.y, org.z, dir.x, dir.y, dir.z, 0.0, t, td, su, sv);
+        // the reference passes the FP32 distance and u, v through Embree's hit record
+        su = double(float(su));
+        sv = double(float(sv));
+        vx.pos = org + dir * double(float(td));
+        ng = normalize(vx.pos - c);
         D3 dpdu{-r * sin(su) * sin(sv), r * cos(su) * sin(sv), 0.0};
         D3 tg = normalize(dpdu - ng * dot(ng, dpdu));
         sf = Frame{tg, normalize(cross(ng, tg)), ng};

@@ -119,14 +119,15 @@ __device__ __forceinline__ bool tri_test(const RayPre &r, float4 a4, float4 b4, 
     return true;
 }
 
-// Sphere user primitive (src/shapes/sphere.inl:12-106): stable quadratic in
-// FP64, nearest root in [tmin, tmax).  u, v carry azimuth/2pi, elevation/pi.
-__device__ __forceinline__ bool sphere_test(const RayPre &r, float dx, float dy, float dz, float4 c4, float rad,
-                                            float tmin, float tmax, float &t_out, float &u_out, float &v_out) {
-    double vx = double(r.ox) - c4.x, vy = double(r.oy) - c4.y, vz = double(r.oz) - c4.z;
-    double A = double(dx) * dx + double(dy) * dy + double(dz) * dz;
-    double B = 2 * (double(dx) * vx + double(dy) * vy + double(dz) * vz);
-    double Cc = vx * vx + vy * vy + vz * vz - double(rad) * rad;
+// Sphere user primitive (src/shapes/sphere.inl:12-106): the stable quadratic
+// in FP64 on the FP32 ray and the FP64 centre and radius, nearest root in
+// [tmin, tmax).
+__device__ __forceinline__ bool sphere_roots(double ox, double oy, double oz, double dx, double dy, double dz,
+                                             double4 c, double tmin, double tmax, double &t_out) {
+    double vx = ox - c.x, vy = oy - c.y, vz = oz - c.z;
+    double A = dx * dx + dy * dy + dz * dz;
+    double B = 2 * (dx * vx + dy * vy + dz * vz);
+    double Cc = vx * vx + vy * vy + vz * vz - c.w * c.w;
     double t0, t1;
     if (A == 0) {
         if (B == 0) return false;
@@ -152,22 +153,25 @@ __device__ __forceinline__ bool sphere_test(const RayPre &r, float dx, float dy,
     if (t0 >= tmin && t0 < tmax) t = t0;
     if (t1 >= tmin && t1 < tmax && t < 0) t = t1;
     if (!(t >= tmin && t < tmax)) return false;
-    double px = r.ox + t * dx - c4.x, py = r.oy + t * dy - c4.y, pz = r.oz + t * dz - c4.z;
-    double el = acos(clampd(py / rad, -1.0, 1.0));
-    double az = atan2(pz / rad, px / rad);
-    t_out = float(t);
-    u_out = float(az * kInvTwoPi);
-    v_out = float(el * kInvPi);
+    t_out = t;
     return true;
 }
 
+struct TraceScene {
+    const BvhNode *__restrict__ nodes;
+    const float4 *__restrict__ prims;    // 3 per leaf slot
+    const double4 *__restrict__ spheres; // centre xyz, radius
+};
+
 // Trace one ray.  stack: this lane's LDS column base (entries at stack[k*stride]);
-// spill: this lane's global spill column.  Returns true on a hit.
-template <bool ANY, bool STATS>
-__device__ __forceinline__ bool trace_ray(const BvhNode *__restrict__ nodes, const float4 *__restrict__ prims,
-                                          float ox, float oy, float oz, float dx, float dy, float dz, float tmin,
-                                          float tmax, int *stack, int stride, int *spill, int spill_stride,
-                                          TraceHit &hit, uint32_t &n_nodes, uint32_t &n_prims) {
+// spill: this lane's global spill column.  Returns true on a hit.  SPH: the
+// scene has sphere primitives (otherwise every leaf slot is a triangle).
+template <bool ANY, bool STATS, bool SPH>
+__device__ __forceinline__ bool trace_ray(const TraceScene &ts, float ox, float oy, float oz, float dx, float dy,
+                                          float dz, float tmin, float tmax, int *stack, int stride, int *spill,
+                                          int spill_stride, TraceHit &hit, uint32_t &n_nodes, uint32_t &n_prims) {
+    const BvhNode *__restrict__ nodes = ts.nodes;
+    const float4 *__restrict__ prims = ts.prims;
     RayPre r = ray_setup(ox, oy, oz, dx, dy, dz);
     int sp = 0;
     int node = 0;  // root is always an inner node
@@ -203,14 +207,16 @@ __device__ __forceinline__ bool trace_ray(const BvhNode *__restrict__ nodes, con
             for (int k = 0; k < cnt; k++) {
                 int slot = first + k;
                 float4 p0 = prims[3 * slot], p1 = prims[3 * slot + 1];
-                float t, u, v;
+                float t, u = 0, v = 0;
                 bool ok;
                 if (STATS) n_prims++;
-                if (__float_as_int(p1.w) == kPrimTri) {
+                if (!SPH || __float_as_int(p1.w) == kPrimTri) {
                     float4 p2 = prims[3 * slot + 2];
                     ok = tri_test(r, p0, p1, p2, tmin, tmax, t, u, v);
                 } else {
-                    ok = sphere_test(r, dx, dy, dz, p0, p1.x, tmin, tmax, t, u, v);
+                    double td;
+                    ok = sphere_roots(ox, oy, oz, dx, dy, dz, ts.spheres[__float_as_int(p1.x)], tmin, tmax, td);
+                    t = float(td);
                 }
                 if (ok) {
                     found = true;
@@ -227,7 +233,38 @@ __device__ __forceinline__ bool trace_ray(const BvhNode *__restrict__ nodes, con
         sp--;
         node = sp < kStackLds ? stack[sp * stride] : spill[(sp - kStackLds) * spill_stride];
     }
+    if (!ANY && found) {
+        float4 p0 = prims[3 * hit.slot], p1 = prims[3 * hit.slot + 1];
+        if (!SPH || __float_as_int(p1.w) == kPrimTri) {
+            // The FP32 distance loses relative accuracy when the hit is close
+            // to the origin next to a large triangle (T cancels); recompute
+            // it once in FP64 from the triangle's plane.
+            float4 p2 = prims[3 * hit.slot + 2];
+            double e1x = double(p1.x) - p0.x, e1y = double(p1.y) - p0.y, e1z = double(p1.z) - p0.z;
+            double e2x = double(p2.x) - p0.x, e2y = double(p2.y) - p0.y, e2z = double(p2.z) - p0.z;
+            double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+            double den = nx * dx + ny * dy + nz * dz;
+            if (den != 0.0) {
+                double num = nx * (double(p0.x) - ox) + ny * (double(p0.y) - oy) + nz * (double(p0.z) - oz);
+                hit.t = float(num / den);
+            }
+        }
+        // sphere hits leave u, v to sphere_hit_uv()
+    }
     return found;
+}
+
+// For a sphere hit: the FP64 distance and the spherical coordinates used as
+// u, v (src/shapes/sphere.inl:85-104).
+__device__ __forceinline__ void sphere_hit_uv(double4 c, double ox, double oy, double oz, double dx, double dy,
+                                           double dz, double tmin, float t_hit, double &t, double &u, double &v) {
+    t = double(t_hit);
+    sphere_roots(ox, oy, oz, dx, dy, dz, c, tmin, double(t_hit) * (1 + 1e-6) + 1e-30, t);
+    double px = ox + t * dx - c.x, py = oy + t * dy - c.y, pz = oz + t * dz - c.z;
+    double el = acos(clampd(py / c.w, -1.0, 1.0));
+    double az = atan2(pz / c.w, px / c.w);
+    u = az * kInvTwoPi;
+    v = el * kInvPi;
 }
 
 }  // namespace ljx

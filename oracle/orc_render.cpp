@@ -1101,7 +1101,10 @@ int orc_scene_bounds(void *sp, double *cr) {
 }
 
 // Closest hit of FP32 rays on the FP32 geometry, computed in FP64.
-// min_bary (may be NULL): smallest barycentric of the hit (edge distance).
+// edge_rel (may be NULL): distance from the hit point to the nearest edge of
+//   the hit triangle, divided by the magnitude of the coordinates involved
+//   (max of t, |origin|_inf, |hit|_inf) -- the scale FP32 resolves to ~1e-7;
+//   +inf for spheres and misses.
 // second_t (may be NULL): t of the nearest hit on any other primitive.
 int orc_trace_closest(void *sp, const ljx_ray *rays, size_t n, ljx_hit *hits, int brute_force,
                       int threads, double *min_bary, double *second_t) {
@@ -1115,7 +1118,7 @@ int orc_trace_closest(void *sp, const ljx_ray *rays, size_t n, ljx_hit *hits, in
             ljx_hit &o = hits[i];
             if (!ok) {
                 o = ljx_hit{0, 0, 0, -1, -1, {0, 0, 0}};
-                if (min_bary) min_bary[i] = 0;
+                if (min_bary) min_bary[i] = inf();
                 if (second_t) second_t[i] = inf();
                 continue;
             }
@@ -1128,7 +1131,25 @@ int orc_trace_closest(void *sp, const ljx_ray *rays, size_t n, ljx_hit *hits, in
             o.ng[0] = float(h.ng.x);
             o.ng[1] = float(h.ng.y);
             o.ng[2] = float(h.ng.z);
-            if (min_bary) min_bary[i] = h.min_bary;
+            if (min_bary) {
+                const Shape &sh = s->shapes[p.shape];
+                if (sh.type == LJX_SHAPE_SPHERE) {
+                    min_bary[i] = inf();
+                } else {
+                    V3 hp = r.org + r.dir * h.t;
+                    V3 v[3] = {sh.positions[sh.indices[3 * p.local]], sh.positions[sh.indices[3 * p.local + 1]],
+                               sh.positions[sh.indices[3 * p.local + 2]]};
+                    Real de = inf();
+                    for (int e = 0; e < 3; e++) {
+                        V3 a = v[e], b = v[(e + 1) % 3];
+                        Real len = length(b - a);
+                        if (len > 0) de = std::min(de, length(cross(hp - a, b - a)) / len);
+                    }
+                    auto amax = [](V3 q) { return std::max(std::fabs(q.x), std::max(std::fabs(q.y), std::fabs(q.z))); };
+                    Real scale = std::max(h.t, std::max(amax(r.org), amax(hp)));
+                    min_bary[i] = de / scale;
+                }
+            }
             if (second_t) {
                 // nearest other primitive: re-run with the winner excluded
                 Real best2 = inf();
