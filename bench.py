@@ -27,7 +27,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--spp", type=int, default=0, help="0: the workload's own spp")
+    ap.add_argument("--workload", choices=["terrain_spheres", "nested_media"], default="terrain_spheres",
+                    help="terrain_spheres (default): the headline 1080p large-mesh workload; nested_media: the "
+                         "1024x1024, 128 spp, depth 16 volumetric workload")
     ap.add_argument("--cpu-spp", type=int, default=0, help="spp of the bounded CPU baseline sample (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-paths", type=int, default=0)
@@ -56,7 +59,17 @@ def main():
     from lajolla_public_amd import api, scenes
 
     t0 = time.time()
-    desc = scenes.terrain_spheres(seed=7, spp=args.spp)
+    if args.workload == "terrain_spheres":
+        desc = scenes.terrain_spheres(seed=7, spp=args.spp or 64)
+        workload = ("terrain_spheres: fBm heightfield 1000x500 + 64 icospheres (subdiv 5) + 1024 emissive "
+                    "triangles = 2,308,746 triangles, constant environment, 1920x1080, 64 spp, max depth 8")
+        metric = "Msamples/s (1080p, depth 8)"
+    else:
+        desc = scenes.nested_media(seed=11, g=0.7, spp=args.spp or 128)
+        workload = ("nested_media: 100,352-triangle index-matched sphere (homogeneous medium, HG g=0.7) inside a "
+                    "RoughDielectric sphere with a second medium, 2 area lights, 1024x1024, 128 spp, max depth 16, "
+                    "RR from depth 5, volpath")
+        metric = "Msamples/s (1024x1024, depth 16, volpath)"
     print(f"[bench] scene generated in {time.time() - t0:.1f}s: {desc.num_triangles} triangles", file=sys.stderr,
           flush=True)
     sc = api.Scene(desc)
@@ -108,7 +121,7 @@ def main():
         cpu = cpu_baseline(desc, args.cpu_spp)
 
     line = {
-        "metric": "Msamples/s (1080p, depth 8)",
+        "metric": metric,
         "value": round(value, 3),
         "unit": "Msamples/s",
         "n_gpus": 1,
@@ -121,8 +134,7 @@ def main():
         "dtype": "f32 traversal / f64 shading",
         "data": "synthetic (seeded scene generator)",
         "config": {
-            "workload": "terrain_spheres: fBm heightfield 1000x500 + 64 icospheres (subdiv 5) + 1024 emissive "
-                        "triangles = 2,308,746 triangles, constant environment, 1920x1080, 64 spp, max depth 8",
+            "workload": workload,
             "width": W, "height": H, "spp": spp, "max_depth": desc.options["max_depth"],
             "triangles": desc.num_triangles, "seed": desc.meta["seed"],
         },

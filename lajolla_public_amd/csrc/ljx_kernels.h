@@ -49,6 +49,11 @@ struct PathBufs {
     uint32_t *bucket; // kNumBuckets * capacity
     uint32_t *shq;
     float4 *sh_o, *sh_d, *sh_c;
+    // volumetric path state
+    int *medium;      // current medium id
+    float4 *nee_c;    // nee_p_cache xyz, dir_pdf
+    float4 *mtp;      // multi_trans_pdf rgb, never_scatter (1/0)
+    float4 *sh_x;     // shadow walk: pdf_nee, pdf_dir, NEE stream state (lo, hi bits)
     int *ctr;
     unsigned long long *stat;
     int *spill;
@@ -157,6 +162,9 @@ __global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int c
                 P.hit[pid] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
                 bucket = kBucketMiss;
             }
+            // volumetric paths decide between medium and surface events after
+            // free-flight sampling: one queue
+            if (P.volpath) bucket = 0;
         }
         // one append per distinct bucket present in the wave
         uint64_t pending = __ballot(bucket >= 0);

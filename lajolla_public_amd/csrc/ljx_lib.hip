@@ -788,6 +788,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
             tm.begin(3, ev);
             launch(k_reset_counters, 1, 64, s, P.ctr, 0, C_TOTAL);
             launch(k_raygen, shade_grid, 256, s, S, P, (W % 8 || H % 8) ? 1 : 0);
+            if (vol) launch(k_vol_init, shade_grid, 256, s, S, P);
             tm.end(3, ev);
             int cur = 0;
             for (int b = 0; b < max_bounces; b++) {
@@ -801,11 +802,11 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
                 trace_stage(sc, P, cur, tgrid, s, stats_on, false);
                 tm.end(0, ev);
                 tm.begin(2, ev);
-                if (vol) vol_shade_all(S, P, cur, shade_grid, s);
+                if (vol) vol_shade_all(S, P, cur, shade_grid, s, sc.has_spheres);
                 else shade_all(sc, P, cur, shade_grid, s);
                 tm.end(2, ev);
                 tm.begin(1, ev);
-                if (vol) vol_shadow(S, P, tgrid, s, stats_on);
+                if (vol) vol_shadow(S, P, tgrid, s, stats_on, sc.has_spheres);
                 else trace_stage(sc, P, cur, tgrid, s, stats_on, true);
                 tm.end(1, ev);
                 cur ^= 1;

@@ -59,6 +59,30 @@ def test_cornell_box_statistics_vs_reference_rng():
     assert abs(g_mean.mean() - o.mean()) < 0.02 * o.mean()
 
 
+def test_volpath_beer_lambert_on_device():
+    from test_oracle_volpath import absorbing_scene
+    sigma = 0.5
+    s = absorbing_scene(sigma, spp=4096)
+    img = api.Scene(s).render()
+    n = s.camera["width"]
+    tx = np.tan(np.radians(s.camera["fov"]) / 2)
+    c = (np.arange(n) + 0.5) / n * 2 - 1
+    X, Y = np.meshgrid(c * tx, c * tx)
+    p = np.exp(-sigma * 2.0 * np.sqrt(1 + X ** 2 + Y ** 2))
+    for k, le in enumerate((1.0, 2.0, 3.0)):
+        z = np.abs(img[..., k] - p * le) / (le * np.sqrt(p * (1 - p) / 4096))
+        assert z.max() < 5.5, (k, z.max())
+
+
+def test_volpath_nested_media_sample_matched():
+    desc = scenes.nested_media(seed=11, g=0.7, width=48, height=48, spp=8, max_depth=16, sphere_res=32)
+    g = api.Scene(desc).render()
+    o, _ = po.OracleScene(desc).render(plan=1)
+    assert np.isfinite(g).all()
+    sample_matched(g, o, frac=0.90)
+    assert abs(g.mean() - o.mean()) < 0.03 * o.mean()
+
+
 def test_terrain_spheres_small_sample_matched():
     desc = scenes.terrain_spheres(seed=7, width=96, height=54, spp=4, max_depth=8, grid=(200, 100), n_spheres=8,
                                   sphere_subdiv=3, n_lights=64)
