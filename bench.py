@@ -115,6 +115,8 @@ def main():
     alg_bytes = ext_rays * bytes_per_ray
     achieved = alg_bytes / (ext_ms / 1000) / 1e9 if ext_ms > 0 else 0.0
     kernel_ms = {k: sum(s[k] for s in stats) / args.steps for k in ("extend_ms", "shadow_ms", "shade_ms", "other_ms")}
+    alg_per_launch = alg_bytes / max(1, ext_launches)
+    traffic = measured_traffic(args.workload)
 
     cpu = None
     if not args.no_cpu:
@@ -148,7 +150,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "profiles/pmc_k_extend_terrain_spheres.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE "
+                              "x1.0, scaled by a copy kernel of known size)" if traffic else None,
+            "algorithmic_bytes_per_launch": round(alg_per_launch),
+            "traffic_over_algorithmic": round(traffic / alg_per_launch, 3) if traffic else None,
             "bytes_per_ray": round(bytes_per_ray, 1),
             "nodes_per_ray": round(nodes_per_ray, 2),
             "prims_per_ray": round(tris_per_ray, 2),
@@ -160,6 +166,18 @@ def main():
     if cpu and cpu.get("value"):
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(line), flush=True)
+
+
+def measured_traffic(workload):
+    """HBM bytes per k_extend launch from the committed rocprofv3 PMC summary
+    of this workload, or None if there is none."""
+    path = os.path.join(ROOT, "profiles", f"pmc_k_extend_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch k_extend:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def cpu_baseline(desc, cpu_spp):
