@@ -271,6 +271,10 @@ typedef struct ljx_render_stats {
     double build_ms;               /* scene upload + BVH build of this scene */
     int32_t bvh_depth;
     int32_t pad;
+    double guide_ms;               /* HIP-event time in guide inference (guided renders) */
+    double train_ms;               /* sample collection + Adam steps (guided renders) */
+    uint64_t train_samples;        /* training samples recorded */
+    uint64_t train_steps;          /* Adam steps taken */
 } ljx_render_stats;
 
 /* Render-call switches that do not change results. */
@@ -346,6 +350,79 @@ int ljx_trace_any_device(void *scene, const ljx_ray *rays, size_t n,
  * src/material.cpp:90-119. */
 int ljx_bsdf_batch(void *scene, const ljx_bsdf_query *queries, size_t n,
                    ljx_bsdf_result *results);
+
+/* ---- learned path guiding ------------------------------------------------
+ * The reference has no path guiding: these calls extend the render path with
+ * a learned directional proposal that is mixed with BSDF sampling by
+ * one-sample MIS (the estimator stays unbiased).  The network maps a surface
+ * vertex (position, normal, outgoing direction) to a mixture of 8 von
+ * Mises-Fisher lobes; LJX_GUIDE_NUM_PARAMS FP32 master parameters in the
+ * canonical layout: per layer l = 0..4, W_l[in][out] row-major, then b_l[out],
+ * with (in, out) = (160, 64), (64, 64) x 3, (64, 40).  Output o = 5 * lobe +
+ * k: k = 0..2 mean direction (normalised), 3 log kappa (clamped to [-3, 8]),
+ * 4 mixture logit (softmax over lobes).
+ *
+ * Features (8 floats per sample, the last unused), each in [0, 1]: position
+ * normalised to the scene box, normal (theta / pi, phi / 2 pi), outgoing
+ * direction (theta / pi, phi / 2 pi); directions use theta = acos(z),
+ * phi = atan2(y, x) wrapped to [0, 2 pi).
+ *
+ * Training minimises -sum_i w_i log q(dir_i | x_i) / sum_i w_i with Adam;
+ * w_i is the sample's radiance-weighted contribution divided by the pdf it
+ * was sampled with, so q approaches the product of BSDF, cosine and incident
+ * radiance. */
+#define LJX_GUIDE_NUM_PARAMS 25384
+#define LJX_GUIDE_NUM_OUTPUTS 40
+
+typedef struct ljx_guide_config {
+    float learning_rate;           /* Adam step size (1e-3) */
+    float beta1;                   /* 0.9 */
+    float beta2;                   /* 0.999 */
+    float epsilon;                 /* 1e-8 */
+    int32_t batch_size;            /* samples per Adam step (1 << 20) */
+    int32_t pad;
+    uint64_t seed;                 /* parameter initialisation */
+} ljx_guide_config;
+
+/* Guided rendering: the first train_spp samples per pixel are rendered in
+ * passes of train_spp_per_pass, each followed by steps_per_pass Adam steps on
+ * the samples it recorded; the remaining samples use the trained network.
+ * All samples enter the image with equal weight. */
+typedef struct ljx_guide_options {
+    float mix_prob;                /* probability of sampling the guide (0.5) */
+    int32_t train_spp;             /* 0: no training (the network is used as is) */
+    int32_t train_spp_per_pass;    /* 1 */
+    int32_t steps_per_pass;        /* Adam steps after each training pass */
+    int32_t max_record_depth;      /* vertices per path recorded for training (<= 16) */
+    int32_t pad;
+} ljx_guide_options;
+
+/* Creates a network with seeded initial parameters on the device. */
+int ljx_guide_create(int device, const ljx_guide_config *cfg, void **guide_out);
+int ljx_guide_destroy(void *guide);
+/* Copies the FP32 master parameters (n must be LJX_GUIDE_NUM_PARAMS). */
+int ljx_guide_get_params(void *guide, float *params, size_t n);
+/* Replaces the master parameters and clears the Adam state. */
+int ljx_guide_set_params(void *guide, const float *params, size_t n);
+/* Network outputs (n x 40, canonical order, before normalisation and
+ * softmax) for n feature rows (n x 8 floats); device pointers. */
+int ljx_guide_forward_device(void *guide, const float *features, size_t n, float *raw_out);
+/* Gradient of the normalised loss with respect to the master parameters for
+ * a batch (features n x 8, dirs n x 4 = direction xyz and weight w); device
+ * pointers.  grad_out: LJX_GUIDE_NUM_PARAMS floats; loss_out may be NULL.
+ * n must not exceed the configured batch_size.  The parameters are not
+ * changed. */
+int ljx_guide_gradient_device(void *guide, const float *features, const float *dirs, size_t n,
+                              float *grad_out, double *loss_out);
+/* One Adam step on a batch (same layout as ljx_guide_gradient_device). */
+int ljx_guide_train_step_device(void *guide, const float *features, const float *dirs, size_t n,
+                                double *loss_out);
+/* Renders like ljx_render_device, sampling directions at surface vertices
+ * from the mixture of the BSDF and the network (path integrator only), and
+ * trains the network on the fly as gopts says.  stats may be NULL. */
+int ljx_render_guided_device(void *scene, void *guide, const ljx_render_options *opts,
+                             const ljx_exec_options *exec, const ljx_guide_options *gopts,
+                             float *rgb_out_device, ljx_render_stats *stats);
 
 /* Device memory helpers for callers without their own allocator. */
 int ljx_device_alloc(int device, size_t bytes, void **ptr);

@@ -199,6 +199,10 @@ class RenderStats(C.Structure):
         ("build_ms", C.c_double),
         ("bvh_depth", C.c_int32),
         ("pad", C.c_int32),
+        ("guide_ms", C.c_double),
+        ("train_ms", C.c_double),
+        ("train_samples", C.c_uint64),
+        ("train_steps", C.c_uint64),
     ]
 
 
@@ -209,6 +213,33 @@ class ExecOptions(C.Structure):
         ("max_paths_in_flight", C.c_int32),
         ("block_size", C.c_int32),
         ("sort_threshold", C.c_int32),
+    ]
+
+
+GUIDE_NUM_PARAMS = 25384
+GUIDE_NUM_OUTPUTS = 40
+
+
+class GuideConfig(C.Structure):
+    _fields_ = [
+        ("learning_rate", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("epsilon", C.c_float),
+        ("batch_size", C.c_int32),
+        ("pad", C.c_int32),
+        ("seed", C.c_uint64),
+    ]
+
+
+class GuideOptions(C.Structure):
+    _fields_ = [
+        ("mix_prob", C.c_float),
+        ("train_spp", C.c_int32),
+        ("train_spp_per_pass", C.c_int32),
+        ("steps_per_pass", C.c_int32),
+        ("max_record_depth", C.c_int32),
+        ("pad", C.c_int32),
     ]
 
 
@@ -232,6 +263,15 @@ SIGNATURES = {
     "ljx_trace_any_device": ([VP, VP, C.c_size_t, VP, C.POINTER(ExecOptions),
                               C.POINTER(C.c_double), C.POINTER(RenderStats)], C.c_int),
     "ljx_bsdf_batch": ([VP, C.POINTER(BSDFQuery), C.c_size_t, C.POINTER(BSDFResult)], C.c_int),
+    "ljx_guide_create": ([C.c_int, C.POINTER(GuideConfig), C.POINTER(VP)], C.c_int),
+    "ljx_guide_destroy": ([VP], C.c_int),
+    "ljx_guide_get_params": ([VP, C.POINTER(C.c_float), C.c_size_t], C.c_int),
+    "ljx_guide_set_params": ([VP, C.POINTER(C.c_float), C.c_size_t], C.c_int),
+    "ljx_guide_forward_device": ([VP, VP, C.c_size_t, VP], C.c_int),
+    "ljx_guide_gradient_device": ([VP, VP, VP, C.c_size_t, VP, C.POINTER(C.c_double)], C.c_int),
+    "ljx_guide_train_step_device": ([VP, VP, VP, C.c_size_t, C.POINTER(C.c_double)], C.c_int),
+    "ljx_render_guided_device": ([VP, VP, C.POINTER(RenderOptions), C.POINTER(ExecOptions),
+                                  C.POINTER(GuideOptions), VP, C.POINTER(RenderStats)], C.c_int),
     "ljx_device_alloc": ([C.c_int, C.c_size_t, C.POINTER(VP)], C.c_int),
     "ljx_device_free": ([VP], C.c_int),
     "ljx_copy_to_device": ([VP, VP, C.c_size_t], C.c_int),

@@ -128,3 +128,103 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+
+
+class Guide:
+    """Learned path-guiding network on the device (no reference counterpart:
+    an extension of the render path).  Parameters are FP32 in the canonical
+    layout of include/lajolla_amd.h; the device computes in BF16 on the matrix
+    cores with FP32 accumulation."""
+
+    NUM_PARAMS = capi.GUIDE_NUM_PARAMS
+
+    def __init__(self, device=0, learning_rate=1e-3, batch_size=1 << 20, seed=1, beta1=0.9, beta2=0.999,
+                 epsilon=1e-8):
+        self.lib = capi.load()
+        cfg = capi.GuideConfig()
+        cfg.learning_rate = learning_rate
+        cfg.beta1 = beta1
+        cfg.beta2 = beta2
+        cfg.epsilon = epsilon
+        cfg.batch_size = batch_size
+        cfg.seed = seed
+        h = C.c_void_p()
+        capi.check(self.lib.ljx_guide_create(device, C.byref(cfg), C.byref(h)), self.lib)
+        self.h = h
+        self.device = device
+        self.batch_size = batch_size
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ljx_guide_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.NUM_PARAMS, np.float32)
+        capi.check(self.lib.ljx_guide_get_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float)), p.size), self.lib)
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        capi.check(self.lib.ljx_guide_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float)), p.size), self.lib)
+
+    def _upload(self, arr):
+        arr = np.ascontiguousarray(arr, np.float32)
+        b = DeviceBuffer(max(arr.nbytes, 16), self.device)
+        b.upload(arr)
+        return b
+
+    def forward(self, feats):
+        """Raw outputs (n, 40) for feature rows (n, 8)."""
+        n = len(feats)
+        f = self._upload(feats)
+        out = DeviceBuffer(n * 40 * 4, self.device)
+        capi.check(self.lib.ljx_guide_forward_device(self.h, f.ptr, n, out.ptr), self.lib)
+        return out.download(np.zeros((n, 40), np.float32))
+
+    def gradient(self, feats, dirs):
+        """(normalised loss, gradient) for a batch; the parameters do not change."""
+        n = len(feats)
+        f, d = self._upload(feats), self._upload(dirs)
+        g = DeviceBuffer(self.NUM_PARAMS * 4, self.device)
+        loss = C.c_double()
+        capi.check(self.lib.ljx_guide_gradient_device(self.h, f.ptr, d.ptr, n, g.ptr, C.byref(loss)), self.lib)
+        return loss.value, g.download(np.zeros(self.NUM_PARAMS, np.float32))
+
+    def train_step(self, feats, dirs):
+        n = len(feats)
+        f, d = self._upload(feats), self._upload(dirs)
+        loss = C.c_double()
+        capi.check(self.lib.ljx_guide_train_step_device(self.h, f.ptr, d.ptr, n, C.byref(loss)), self.lib)
+        return loss.value
+
+
+def guide_options(mix_prob=0.5, train_spp=0, train_spp_per_pass=1, steps_per_pass=4, max_record_depth=0):
+    o = capi.GuideOptions()
+    o.mix_prob = mix_prob
+    o.train_spp = train_spp
+    o.train_spp_per_pass = train_spp_per_pass
+    o.steps_per_pass = steps_per_pass
+    o.max_record_depth = max_record_depth
+    return o
+
+
+def render_guided(scene: Scene, guide: Guide, options=None, exec_options=None, gopts=None, stats=False):
+    """Guided render into host memory (the image is produced in HBM and copied once)."""
+    W, H = scene.width, scene.height
+    out = DeviceBuffer(W * H * 3 * 4, guide.device)
+    st = capi.RenderStats()
+    g = gopts if gopts is not None else guide_options()
+    capi.check(scene.lib.ljx_render_guided_device(scene.h, guide.h, C.byref(options) if options is not None else None,
+                                                  C.byref(exec_options) if exec_options is not None else None,
+                                                  C.byref(g), out.ptr, C.byref(st)), scene.lib)
+    img = out.download(np.zeros((H, W, 3), np.float32))
+    if stats:
+        return img, stats_dict(st)
+    return img

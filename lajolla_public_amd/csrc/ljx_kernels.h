@@ -19,6 +19,7 @@
 // shape or queue order.
 #pragma once
 
+#include "ljx_guide.h"
 #include "ljx_shade.h"
 
 namespace ljx {
@@ -66,7 +67,25 @@ struct PathBufs {
     int volpath;
     int max_null;
     uint64_t seed;
+    // learned guiding (path integrator)
+    float4 *glob;      // capacity x 8 lobes: kappa * mean xyz, log constant (pdf = exp(dot + c))
+    float4 *gdir;      // guide-sampled direction xyz; w: 1 from the guide, 0 from the BSDF, -1 not guided
+    float4 *rec_feat;  // rec_depth x capacity x 2: network features of each vertex
+    float4 *rec_dir;   // rec_depth x capacity: sampled direction xyz; w: 1 if a training sample
+    float4 *rec_beta;  // rec_depth x capacity: throughput at the vertex
+    float4 *rec_c;     // rec_depth x capacity: radiance the vertex adds (emission, NEE, environment)
+    int rec_depth;     // vertices recorded per path (0: not recording)
+    uint32_t guide_mask;  // material types sampled from the mixture
+    float guide_mix;   // probability of the guide in the mixture
+    float box_lo[3], box_scale[3];  // position features: (p - lo) * scale
 };
+
+// Shadow-queue entries carry the path slot in the low 24 bits and the vertex
+// index in the high 8.
+constexpr int kSlotBits = 24;
+constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1;
+// Material types never guided: their lobes are near-specular transmission.
+__host__ __device__ constexpr bool guidable(int mt) { return mt != 2 && mt != 5; }
 
 __device__ __forceinline__ void slot_pixel(const PathBufs &P, int i, int &s, int &x, int &y, bool &valid) {
     s = P.pass_s0 + i / P.pass_nq;
@@ -184,7 +203,7 @@ __global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int c
 }
 
 // Pending NEE contribution and the next ray of one path vertex.
-template <int MT, bool SPH>
+template <int MT, bool SPH, bool GUIDE>
 __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBufs &P, uint32_t pid, bool &push_ext,
                                               bool &push_sh, float4 &sh_o, float4 &sh_d, float4 &sh_c) {
     float4 h = P.hit[pid];
@@ -210,6 +229,19 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
     bool alive = true;
     push_ext = false;
     push_sh = false;
+    // guided vertex: directions come from the mixture (1 - a) BSDF + a guide
+    bool guided = false;
+    float4 gd = make_float4(0.f, 0.f, 0.f, -1.f);
+    const float4 *lobes = nullptr;
+    const double alpha = GUIDE ? double(P.guide_mix) : 0.0;
+    if (GUIDE && guidable(MT)) {
+        gd = P.gdir[pid];
+        guided = gd.w >= 0.f;
+        lobes = P.glob + size_t(pid) * guide::kLobes;
+    }
+    const bool rec = GUIDE && b < P.rec_depth;
+    const size_t rix = size_t(b) * P.capacity + pid;
+    D3 rad_in = rad;
     if (b == 0) {
         // path_tracing.h:58-61
         if (v.area_light_id >= 0)
@@ -237,6 +269,10 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
             if (pcg_double(st, inc) > rr) alive = false;
         }
         if (alive) tp = tp * (G * f) / (p2 * rr);
+    }
+    if (rec) {
+        D3 add = rad - rad_in;
+        P.rec_c[rix] = make_float4(float(add.x), float(add.y), float(add.z), 0.f);
     }
     if (alive && !(P.max_depth == -1 || b + 3 <= P.max_depth + 1)) alive = false;
     if (alive && S.num_lights == 0) {
@@ -268,7 +304,11 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
             D3 f = bsdf_eval<MT>(mat, dir_view, dir_light, v.g);
             D3 L = light_emission(l, -dir_light, lp);
             D3 C1 = G * f * L;
-            double p2 = bsdf_pdf<MT>(mat, dir_view, dir_light, v.g) * G;
+            double pb = bsdf_pdf<MT>(mat, dir_view, dir_light, v.g);
+            if (guided)
+                pb = (1 - alpha) * pb +
+                     alpha * double(guide::mixture_pdf(lobes, float(dir_light.x), float(dir_light.y), float(dir_light.z)));
+            double p2 = pb * G;
             double w1 = (p1 * p1) / (p1 * p1 + p2 * p2);
             C1 = C1 / p1;
             D3 c = tp * C1 * w1;
@@ -276,18 +316,35 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
                 push_sh = true;
                 sh_o = make_float4(float(v.pos.x), float(v.pos.y), float(v.pos.z), float(S.eps));
                 sh_d = make_float4(float(dir_light.x), float(dir_light.y), float(dir_light.z), float(tfar));
-                sh_c = make_float4(float(c.x), float(c.y), float(c.z), __int_as_float(int(pid)));
+                sh_c = make_float4(float(c.x), float(c.y), float(c.z),
+                                   __int_as_float(int(pid | (uint32_t(min(b, 255)) << kSlotBits))));
             }
         }
         // BSDF sampling (path_tracing.h:209-259)
         double bu0 = pcg_double(st, inc), bu1 = pcg_double(st, inc), bw = pcg_double(st, inc);
-        BsdfSample bs = bsdf_sample<MT>(mat, dir_view, v.g, bu0, bu1, bw);
+        BsdfSample bs;
+        if (guided && gd.w > 0.5f) {
+            // drawn from the guide by k_guide_infer; the BSDF draws above keep
+            // the path's stream aligned with the unguided tracer
+            bs.valid = true;
+            bs.dir = normalize(D3{double(gd.x), double(gd.y), double(gd.z)});
+            bs.eta = 0;
+        } else {
+            bs = bsdf_sample<MT>(mat, dir_view, v.g, bu0, bu1, bw);
+        }
         if (!bs.valid) {
             alive = false;
         } else {
             if (bs.eta != 0) eta_scale /= (bs.eta * bs.eta);
             D3 f = bsdf_eval<MT>(mat, dir_view, bs.dir, v.g);
             double pdf = bsdf_pdf<MT>(mat, dir_view, bs.dir, v.g);
+            if (guided) {
+                // one-sample MIS over the two strategies (balance heuristic):
+                // the path weight is f / mixture pdf
+                pdf = (1 - alpha) * pdf +
+                      alpha * double(guide::mixture_pdf(lobes, float(bs.dir.x), float(bs.dir.y), float(bs.dir.z)));
+                if (f.x == 0 && f.y == 0 && f.z == 0) pdf = 0;  // nothing more can be added
+            }
             if (pdf <= 0) {
                 alive = false;
             } else {
@@ -296,6 +353,10 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
                 P.ray_d[pid] = make_float4(float(bs.dir.x), float(bs.dir.y), float(bs.dir.z), __builtin_huge_valf());
                 P.depth[pid] = b + 1;
                 push_ext = true;
+                if (rec && guided) {
+                    P.rec_dir[rix] = make_float4(float(bs.dir.x), float(bs.dir.y), float(bs.dir.z), 1.f);
+                    P.rec_beta[rix] = make_float4(float(tp.x), float(tp.y), float(tp.z), 0.f);
+                }
             }
         }
     }
@@ -304,7 +365,7 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
     P.rad[pid] = make_float4(float(rad.x), float(rad.y), float(rad.z), 0.f);
 }
 
-template <int MT, bool SPH>
+template <int MT, bool SPH, bool GUIDE>
 __global__ __launch_bounds__(256) void k_shade(DeviceScene S, PathBufs P, int cur) {
     int n = P.ctr[C_BUCKET + MT];
     const uint32_t *q = P.bucket + size_t(MT) * P.capacity;
@@ -317,7 +378,7 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene S, PathBufs P, int cu
         uint32_t pid = 0;
         if (i < n) {
             pid = q[i];
-            shade_surface<MT, SPH>(S, P, pid, pe, ps, so, sd, sc);
+            shade_surface<MT, SPH, GUIDE>(S, P, pid, pe, ps, so, sd, sc);
         }
         int k = wave_append(pe, &P.ctr[C_EXT + (cur ^ 1)]);
         if (k >= 0) next[k] = pid;
@@ -347,7 +408,9 @@ __global__ __launch_bounds__(256) void k_miss(DeviceScene S, PathBufs P) {
         float4 r4 = P.rad[pid];
         D3 rad{r4.x, r4.y, r4.z};
         D3 L = light_emission(env, -dir, LightPoint{{0, 0, 0}, {0, 0, 0}});
-        if (P.depth[pid] == 0) {
+        D3 rad_in = rad;
+        int b = P.depth[pid];
+        if (b == 0) {
             rad = rad + tp * L;
         } else {
             float4 pv = P.prev[pid];
@@ -361,6 +424,10 @@ __global__ __launch_bounds__(256) void k_miss(DeviceScene S, PathBufs P) {
             rad = rad + tp * C2 * w2;
         }
         P.rad[pid] = make_float4(float(rad.x), float(rad.y), float(rad.z), 0.f);
+        if (b < P.rec_depth) {
+            D3 add = rad - rad_in;
+            P.rec_c[size_t(b) * P.capacity + pid] = make_float4(float(add.x), float(add.y), float(add.z), 0.f);
+        }
     }
 }
 
@@ -384,9 +451,16 @@ __global__ __launch_bounds__(256) void k_shadow(DeviceScene S, PathBufs P) {
                                           stack + threadIdx.x, blockDim.x, spill, stride, h, nn, np);
         if (!occ) {
             float4 c = P.sh_c[i];
-            int pid = __float_as_int(c.w);
+            uint32_t bits = uint32_t(__float_as_int(c.w));
+            uint32_t pid = bits & kSlotMask;
             float4 r = P.rad[pid];
             P.rad[pid] = make_float4(r.x + c.x, r.y + c.y, r.z + c.z, 0.f);
+            int b = int(bits >> kSlotBits);
+            if (b < P.rec_depth) {
+                float4 *rc = &P.rec_c[size_t(b) * P.capacity + pid];
+                float4 o = *rc;
+                *rc = make_float4(o.x + c.x, o.y + c.y, o.z + c.z, 0.f);
+            }
         }
     }
     if (STATS) {

@@ -2,6 +2,9 @@
 // upload, BVH build, the wavefront render loop and the parity entry points.
 #include "../../include/lajolla_amd.h"
 #include "ljx_bvh.h"
+#include "ljx_guide_net.h"
+#include "ljx_guide_render.h"
+#include "ljx_host.h"
 #include "ljx_kernels.h"
 #include "ljx_vol.h"
 
@@ -20,40 +23,6 @@
 using namespace ljx;
 
 namespace {
-
-thread_local std::string g_last_error;
-
-struct LjxFail {
-    int code;
-    std::string msg;
-};
-
-#define HIPX(call)                                                                              \
-    do {                                                                                        \
-        hipError_t e_ = (call);                                                                 \
-        if (e_ != hipSuccess)                                                                   \
-            throw LjxFail{e_ == hipErrorOutOfMemory ? LJX_ERR_OUT_OF_MEMORY : LJX_ERR_HIP,      \
-                          std::string(#call) + ": " + hipGetErrorString(e_)};                   \
-    } while (0)
-
-void fail(int code, const std::string &msg) { throw LjxFail{code, msg}; }
-
-template <class F>
-int guarded(F f) {
-    try {
-        f();
-        return LJX_OK;
-    } catch (const LjxFail &e) {
-        g_last_error = e.msg;
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        g_last_error = "host allocation failed";
-        return LJX_ERR_OUT_OF_MEMORY;
-    } catch (const std::exception &e) {
-        g_last_error = e.what();
-        return LJX_ERR_INVALID_ARGUMENT;
-    }
-}
 
 struct DevBuf {
     void *p = nullptr;
@@ -121,11 +90,27 @@ void make_dist(const std::vector<double> &f, std::vector<double> &pmf, std::vect
 }
 
 // --------------------------------------------------------------- the scene
+// Buffers of guided rendering, allocated on the first guided render.
+struct GuideCtx {
+    int capacity = 0, rec_depth = 0, sample_cap = 0;
+    float4 *glob = nullptr, *gdir = nullptr, *rec_feat = nullptr, *rec_dir = nullptr, *rec_beta = nullptr,
+           *rec_c = nullptr;
+    float4 *s_feat = nullptr, *s_dir = nullptr;  // training samples of the last recording pass
+    int *s_count = nullptr;
+    std::vector<DevBuf> bufs;
+};
+
 struct RenderCtx {
     int capacity = 0;
     int trace_grid = 0;
     PathBufs P{};
     std::vector<DevBuf> bufs;
+    std::unique_ptr<GuideCtx> gc;
+    ~RenderCtx() {
+        if (gc)
+            for (auto &b : gc->bufs)
+                if (b.p) hipFree(b.p);
+    }
 };
 
 struct Scene {
@@ -142,6 +127,7 @@ struct Scene {
     bool has_media = false;
     double bound_center[3] = {0, 0, 0};
     double bound_radius = 0;
+    float box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};  // axis-aligned bounds of every primitive
     int bvh_depth = 0;
     double build_ms = 0;
     std::unique_ptr<RenderCtx> rc;
@@ -353,6 +339,10 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
     double dx = double(hi[0]) - lo[0], dy = double(hi[1]) - lo[1], dz = double(hi[2]) - lo[2];
     sc.bound_radius = std::sqrt(dx * dx + dy * dy + dz * dz) / 2;
     for (int k = 0; k < 3; k++) sc.bound_center[k] = (double(lo[k]) + double(hi[k])) / 2;
+    for (int k = 0; k < 3; k++) {
+        sc.box_lo[k] = lo[k];
+        sc.box_hi[k] = hi[k];
+    }
     S.bound_radius = sc.bound_radius;
     S.eps = std::min(sc.bound_radius * 1e-5, 0.01);
 
@@ -610,7 +600,7 @@ struct SpillBuf {
 struct Timer {
     bool on;
     hipStream_t s;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];  // extend, shadow, shade, other
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[6];  // extend, shadow, shade, other, guide, train
     void begin(int k, hipEvent_t &a) {
         if (!on) return;
         HIPX(hipEventCreate(&a));
@@ -681,24 +671,29 @@ void launch(K kernel, int grid, int block, hipStream_t s, A... args) {
     HIPX(hipGetLastError());
 }
 
-template <bool SPH>
+template <bool SPH, bool GUIDE>
 void shade_all_t(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s) {
     const DeviceScene &S = sc.ds;
     unsigned m = sc.mat_types_present;
-    if (m & (1u << 0)) launch(k_shade<0, SPH>, grid, 256, s, S, P, cur);
-    if (m & (1u << 1)) launch(k_shade<1, SPH>, grid, 256, s, S, P, cur);
-    if (m & (1u << 2)) launch(k_shade<2, SPH>, grid, 256, s, S, P, cur);
-    if (m & (1u << 3)) launch(k_shade<3, SPH>, grid, 256, s, S, P, cur);
-    if (m & (1u << 4)) launch(k_shade<4, SPH>, grid, 256, s, S, P, cur);
-    if (m & (1u << 5)) launch(k_shade<5, SPH>, grid, 256, s, S, P, cur);
-    if (m & (1u << 6)) launch(k_shade<6, SPH>, grid, 256, s, S, P, cur);
-    if (m & (1u << 7)) launch(k_shade<7, SPH>, grid, 256, s, S, P, cur);
-    if (m & (1u << 8)) launch(k_shade<8, SPH>, grid, 256, s, S, P, cur);
+    if (m & (1u << 0)) launch(k_shade<0, SPH, GUIDE>, grid, 256, s, S, P, cur);
+    if (m & (1u << 1)) launch(k_shade<1, SPH, GUIDE>, grid, 256, s, S, P, cur);
+    if (m & (1u << 2)) launch(k_shade<2, SPH, GUIDE>, grid, 256, s, S, P, cur);
+    if (m & (1u << 3)) launch(k_shade<3, SPH, GUIDE>, grid, 256, s, S, P, cur);
+    if (m & (1u << 4)) launch(k_shade<4, SPH, GUIDE>, grid, 256, s, S, P, cur);
+    if (m & (1u << 5)) launch(k_shade<5, SPH, GUIDE>, grid, 256, s, S, P, cur);
+    if (m & (1u << 6)) launch(k_shade<6, SPH, GUIDE>, grid, 256, s, S, P, cur);
+    if (m & (1u << 7)) launch(k_shade<7, SPH, GUIDE>, grid, 256, s, S, P, cur);
+    if (m & (1u << 8)) launch(k_shade<8, SPH, GUIDE>, grid, 256, s, S, P, cur);
     launch(k_miss, grid, 256, s, S, P);
 }
-void shade_all(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s) {
-    if (sc.has_spheres) shade_all_t<true>(sc, P, cur, grid, s);
-    else shade_all_t<false>(sc, P, cur, grid, s);
+void shade_all(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s, bool guide) {
+    if (guide) {
+        if (sc.has_spheres) shade_all_t<true, true>(sc, P, cur, grid, s);
+        else shade_all_t<false, true>(sc, P, cur, grid, s);
+    } else {
+        if (sc.has_spheres) shade_all_t<true, false>(sc, P, cur, grid, s);
+        else shade_all_t<false, false>(sc, P, cur, grid, s);
+    }
 }
 template <bool STATS, bool SPH>
 void trace_stage_t(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s, bool shadow) {
@@ -715,8 +710,41 @@ void trace_stage(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s, bool 
     }
 }
 
+void ensure_guide(Scene &sc, RenderCtx &rc, int rec_depth) {
+    int cap = rc.capacity;
+    if (rc.gc && rc.gc->capacity == cap && rc.gc->rec_depth >= rec_depth) return;
+    if (rc.gc)
+        for (auto &b : rc.gc->bufs)
+            if (b.p) hipFree(b.p);
+    rc.gc = std::make_unique<GuideCtx>();
+    GuideCtx &g = *rc.gc;
+    g.capacity = cap;
+    g.rec_depth = rec_depth;
+    size_t C = size_t(cap), R = size_t(rec_depth);
+    g.glob = sc.alloc<float4>(C * guide::kLobes, g.bufs);
+    g.gdir = sc.alloc<float4>(C, g.bufs);
+    g.rec_feat = sc.alloc<float4>(2 * R * C, g.bufs);
+    g.rec_dir = sc.alloc<float4>(R * C, g.bufs);
+    g.rec_beta = sc.alloc<float4>(R * C, g.bufs);
+    g.rec_c = sc.alloc<float4>(R * C, g.bufs);
+    g.sample_cap = int(std::min<size_t>(R * C, size_t(1) << 24));
+    g.s_feat = sc.alloc<float4>(2 * size_t(g.sample_cap), g.bufs);
+    g.s_dir = sc.alloc<float4>(size_t(g.sample_cap), g.bufs);
+    g.s_count = sc.alloc<int>(4, g.bufs);
+}
+
+void guide_infer(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s, const guide::Net &net) {
+    if (sc.has_spheres) launch(k_guide_infer<true>, grid, 256, s, sc.ds, P, cur, (const guide::bf16x8 *)net.ffrag,
+                               (const float *)net.fbias);
+    else launch(k_guide_infer<false>, grid, 256, s, sc.ds, P, cur, (const guide::bf16x8 *)net.ffrag,
+                (const float *)net.fbias);
+}
+
+// The wavefront render loop.  With net set (path integrator only), surface
+// directions come from the BSDF / guide mixture, and the first train_spp
+// samples per pixel are recording passes, each followed by Adam steps.
 void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options *ex, float *out_dev,
-                 ljx_render_stats *stats) {
+                 ljx_render_stats *stats, guide::Net *net = nullptr, const ljx_guide_options *go = nullptr) {
     auto t_start = std::chrono::steady_clock::now();
     ljx_render_options opt = o ? *o : sc.options;
     if (opt.samples_per_pixel <= 0) fail(LJX_ERR_INVALID_ARGUMENT, "samples_per_pixel must be positive");
@@ -727,6 +755,21 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
     if (vol && sc.ds.envmap_light_id >= 0)
         fail(LJX_ERR_UNSUPPORTED, "volpath assumes no environment map (handouts/homework2.tex:180)");
     if (opt.max_depth < -1) fail(LJX_ERR_INVALID_ARGUMENT, "max_depth must be >= -1");
+    const bool guided = net != nullptr;
+    ljx_guide_options gopt{};
+    if (guided) {
+        if (vol) fail(LJX_ERR_UNSUPPORTED, "guided rendering supports the path integrator only");
+        if (net->device != sc.device) fail(LJX_ERR_INVALID_ARGUMENT, "guide and scene live on different devices");
+        if (go) gopt = *go;
+        else gopt.mix_prob = 0.5f;
+        if (!(gopt.mix_prob >= 0.f && gopt.mix_prob < 1.f))
+            fail(LJX_ERR_INVALID_ARGUMENT, "mix_prob must be in [0, 1)");
+        if (gopt.train_spp < 0 || gopt.train_spp_per_pass < 0 || gopt.steps_per_pass < 0)
+            fail(LJX_ERR_INVALID_ARGUMENT, "negative guide option");
+        if (gopt.train_spp_per_pass == 0) gopt.train_spp_per_pass = 1;
+        int md = opt.max_depth > 0 ? opt.max_depth : 16;
+        gopt.max_record_depth = std::max(1, std::min(gopt.max_record_depth > 0 ? gopt.max_record_depth : md, 16));
+    }
     ljx_exec_options e{};
     if (ex) e = *ex;
     HIPX(hipSetDevice(sc.device));
@@ -736,7 +779,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
     int npix = ntx8 * nty8 * 64;
     int spp = opt.samples_per_pixel;
     long long want = e.max_paths_in_flight > 0 ? e.max_paths_in_flight : (1 << 22);
-    want = std::max<long long>(want, 64);
+    want = std::min<long long>(std::max<long long>(want, 64), 1 << kSlotBits);
     int capacity;
     int ns_per_pass, nq_per_pass;
     if (npix <= want) {
@@ -759,8 +802,30 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
     P.seed = opt.seed;
     P.volpath = vol ? 1 : 0;
     P.max_null = opt.max_null_collisions;
+    P.rec_depth = 0;
+    GuideCtx *gc = nullptr;
+    if (guided) {
+        ensure_guide(sc, rc, gopt.max_record_depth);
+        gc = rc.gc.get();
+        P.glob = gc->glob;
+        P.gdir = gc->gdir;
+        P.rec_feat = gc->rec_feat;
+        P.rec_dir = gc->rec_dir;
+        P.rec_beta = gc->rec_beta;
+        P.rec_c = gc->rec_c;
+        P.guide_mix = gopt.mix_prob;
+        P.guide_mask = 0;
+        for (int t = 0; t < LJX_MAT_COUNT; t++)
+            if (guidable(t)) P.guide_mask |= 1u << t;
+        for (int k = 0; k < 3; k++) {
+            float ext = sc.box_hi[k] - sc.box_lo[k];
+            P.box_lo[k] = sc.box_lo[k];
+            P.box_scale[k] = ext > 0 ? 1.f / ext : 0.f;
+        }
+    }
     const DeviceScene &S = sc.ds;
     int shade_grid = std::max(1, std::min(capacity / 256 + 1, sc.num_cu * 16));
+    int guide_grid = std::max(1, std::min(capacity / 256 + 1, sc.num_cu * 2));
     int tgrid = rc.trace_grid;
     Timer tm{e.collect_timing != 0, s};
     bool stats_on = e.collect_traversal != 0;
@@ -773,11 +838,15 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
     HIPX(hipGetLastError());
     tm.end(3, ev);
 
-    uint64_t passes = 0;
+    uint64_t passes = 0, train_samples = 0, train_steps = 0;
     int max_bounces = opt.max_depth >= 0 ? std::max(opt.max_depth, 1) : 1 << 20;
     std::vector<int> host_ctr(C_TOTAL);
-    for (int s0 = 0; s0 < spp; s0 += ns_per_pass) {
-        int ns = std::min(ns_per_pass, spp - s0);
+    const int train_spp = guided ? std::min(gopt.train_spp, spp) : 0;
+    for (int s0 = 0; s0 < spp;) {
+        const bool recording = s0 < train_spp;
+        int ns = recording ? std::min({gopt.train_spp_per_pass, train_spp - s0, ns_per_pass})
+                           : std::min(ns_per_pass, spp - s0);
+        P.rec_depth = recording ? gopt.max_record_depth : 0;
         for (int q0 = 0; q0 < npix; q0 += nq_per_pass) {
             int nq = std::min(nq_per_pass, npix - q0);
             P.pass_s0 = s0;
@@ -789,6 +858,9 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
             launch(k_reset_counters, 1, 64, s, P.ctr, 0, C_TOTAL);
             launch(k_raygen, shade_grid, 256, s, S, P, (W % 8 || H % 8) ? 1 : 0);
             if (vol) launch(k_vol_init, shade_grid, 256, s, S, P);
+            if (recording)
+                launch(k_guide_clear, shade_grid, 256, s, P.rec_dir, P.rec_beta, P.rec_c,
+                       size_t(P.rec_depth) * P.capacity);
             tm.end(3, ev);
             int cur = 0;
             for (int b = 0; b < max_bounces; b++) {
@@ -801,9 +873,14 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
                 tm.begin(0, ev);
                 trace_stage(sc, P, cur, tgrid, s, stats_on, false);
                 tm.end(0, ev);
+                if (guided) {
+                    tm.begin(4, ev);
+                    guide_infer(sc, P, cur, guide_grid, s, *net);
+                    tm.end(4, ev);
+                }
                 tm.begin(2, ev);
                 if (vol) vol_shade_all(S, P, cur, shade_grid, s, sc.has_spheres);
-                else shade_all(sc, P, cur, shade_grid, s);
+                else shade_all(sc, P, cur, shade_grid, s, guided);
                 tm.end(2, ev);
                 tm.begin(1, ev);
                 if (vol) vol_shadow(S, P, tgrid, s, stats_on, sc.has_spheres);
@@ -814,7 +891,26 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
             tm.begin(3, ev);
             launch(k_accumulate, std::max(1, nq / 256 + 1), 256, s, P);
             tm.end(3, ev);
+            if (recording) {
+                // training samples of this pass, then Adam steps on them
+                tm.begin(5, ev);
+                launch(k_reset_counters, 1, 64, s, gc->s_count, 0, 1);
+                launch(k_guide_collect, shade_grid, 256, s, P, gc->s_feat, gc->s_dir, gc->s_count, gc->sample_cap);
+                int cnt = 0;
+                HIPX(hipMemcpyAsync(&cnt, gc->s_count, sizeof(int), hipMemcpyDeviceToHost, s));
+                HIPX(hipStreamSynchronize(s));
+                int navail = std::min(cnt, gc->sample_cap);
+                train_samples += uint64_t(navail);
+                for (int k = 0; k < gopt.steps_per_pass && navail > 0; k++) {
+                    long long start = (static_cast<long long>(k) * net->batch_cap) % navail;
+                    int cntk = int(std::min<long long>(net->batch_cap, navail - start));
+                    guide::net_step(*net, gc->s_feat + 2 * start, gc->s_dir + start, cntk, true, s, nullptr);
+                    train_steps++;
+                }
+                tm.end(5, ev);
+            }
         }
+        s0 += ns;
     }
     tm.begin(3, ev);
     size_t nout = size_t(W) * H * 3;
@@ -836,6 +932,10 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
         stats->shadow_ms = tm.total(1);
         stats->shade_ms = tm.total(2);
         stats->other_ms = tm.total(3);
+        stats->guide_ms = tm.total(4);
+        stats->train_ms = tm.total(5);
+        stats->train_samples = train_samples;
+        stats->train_steps = train_steps;
         stats->node_visits = st[S_NODES];
         stats->tri_tests = st[S_PRIMS];
         stats->shadow_node_visits = st[S_SH_NODES];
@@ -845,7 +945,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
         stats->bvh_depth = sc.bvh_depth;
         stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     } else {
-        for (int k = 0; k < 4; k++) tm.total(k);
+        for (int k = 0; k < 6; k++) tm.total(k);
     }
 }
 
@@ -986,6 +1086,16 @@ int ljx_render(void *scene, const ljx_render_options *opts, const ljx_exec_optio
         } f{d};
         render_impl(*sc, opts, exec, d, stats);
         HIPX(hipMemcpy(rgb_out, d, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int ljx_render_guided_device(void *scene, void *guide_h, const ljx_render_options *opts, const ljx_exec_options *exec,
+                             const ljx_guide_options *gopts, float *rgb_out_device, ljx_render_stats *stats) {
+    return guarded([&]() {
+        Scene *sc = as_scene(scene);
+        if (!guide_h) fail(LJX_ERR_INVALID_ARGUMENT, "guide handle is NULL");
+        if (!rgb_out_device) fail(LJX_ERR_INVALID_ARGUMENT, "output is NULL");
+        render_impl(*sc, opts, exec, rgb_out_device, stats, static_cast<guide::Net *>(guide_h), gopts);
     });
 }
 

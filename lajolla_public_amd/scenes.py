@@ -12,6 +12,9 @@ recorded 64-bit seed:
 * ``terrain_spheres``  -- 1920x1080, 64 spp, depth 8, ~2.31M triangles,
                           Lambertian / RoughPlastic / DisneyBSDF, 1024
                           emissive triangles and a constant environment
+* ``terrain_gap``      -- terrain_spheres under a ceiling with a 5% gap and a
+                          small emitter above it carrying 90% of the light
+                          power (the learned-guiding workload)
 * ``nested_media``     -- 1024x1024, 128 spp, depth 16, homogeneous media
                           inside a 100k-triangle sphere nested in a
                           dielectric sphere, Henyey-Greenstein phase
@@ -448,6 +451,64 @@ def terrain_spheres(seed=7, width=1920, height=1080, spp=64, max_depth=8, grid=(
     s.options.update(samples_per_pixel=spp, max_depth=max_depth, rr_depth=5)
     s.meta = dict(seed=seed, grid=list(grid), n_spheres=n_spheres, sphere_subdiv=sphere_subdiv,
                   n_lights=n_lights, env=list(env))
+    return s
+
+
+def terrain_gap(seed=7, width=1920, height=1080, spp=64, max_depth=8, gap_frac=0.05, power_frac=0.9, **kw):
+    """Guiding workload: the terrain_spheres scene under an occluding ceiling
+    (y = 10.5, covering the terrain and the camera) with a square gap of
+    gap_frac of its area, and a small emitter above the gap that carries
+    power_frac of the scene's total light power (lajolla's light power:
+    pi * area * luminance for area lights, pi r^2 * luminance for the
+    environment, src/lights/diffuse_area_light.inl:1-3, src/lights/envmap.inl:2-4).
+    Most of the scene sees the bright emitter only through the gap, i.e. by
+    indirect light from the patch it illuminates."""
+    s = terrain_spheres(seed=seed, width=width, height=height, spp=spp, max_depth=max_depth, **kw)
+    s.name = "terrain_gap"
+    rng = np.random.default_rng(seed + 1000)
+    y = 10.5
+    x0, x1, z0, z1 = -24.0, 24.0, -14.0, 30.0
+    area = (x1 - x0) * (z1 - z0)
+    side = math.sqrt(gap_frac * area)
+    gx, gz = float(rng.uniform(-8, 8)), float(rng.uniform(-5, 3))
+    hx0, hx1, hz0, hz1 = gx - side / 2, gx + side / 2, gz - side / 2, gz + side / 2
+    m_ceil = s.add_material(capi.MAT_LAMBERTIAN, (0.5, 0.5, 0.5))
+    # four quads around the hole, normals facing down
+    for (a0, a1, b0, b1) in ((x0, hx0, z0, z1), (hx1, x1, z0, z1), (hx0, hx1, z0, hz0), (hx0, hx1, hz1, z1)):
+        P, I, N = quad((a0, y, b0), (a1, y, b0), (a1, y, b1), (a0, y, b1))
+        if N[0][1] > 0:
+            P, I, N = quad((a0, y, b0), (a0, y, b1), (a1, y, b1), (a1, y, b0))
+        s.add_mesh(P, I, m_ceil, normals=N)
+    # power of the lights already present
+    lumw = np.array([0.212671, 0.715160, 0.072169])
+    lo = np.min([sh["positions"].min(0) for sh in s.shapes if sh["positions"] is not None], axis=0)
+    hi = np.max([sh["positions"].max(0) for sh in s.shapes if sh["positions"] is not None], axis=0)
+    r = np.linalg.norm(hi - lo) / 2
+    others = 0.0
+    for l in s.lights:
+        if l["type"] == capi.LIGHT_DIFFUSE_AREA:
+            P = s.shapes[l["shape_id"]]["positions"].astype(np.float64)
+            I = s.shapes[l["shape_id"]]["indices"]
+            a = 0.5 * np.linalg.norm(np.cross(P[I[:, 1]] - P[I[:, 0]], P[I[:, 2]] - P[I[:, 0]]), axis=1).sum()
+            others += np.pi * a * float(np.asarray(l["intensity"]) @ lumw)
+        else:
+            others += np.pi * r * r * float(l["env"].reshape(3) @ lumw) * l["scale"]
+    # the bright emitter: a 1 x 1 quad 2.5 above the gap, facing down
+    ls = 1.0
+    ly = y + 2.5
+    P, I, N = quad((gx - ls / 2, ly, gz - ls / 2), (gx - ls / 2, ly, gz + ls / 2), (gx + ls / 2, ly, gz + ls / 2),
+                   (gx + ls / 2, ly, gz - ls / 2))
+    if N[0][1] > 0:
+        P, I, N = quad((gx - ls / 2, ly, gz - ls / 2), (gx + ls / 2, ly, gz - ls / 2), (gx + ls / 2, ly, gz + ls / 2),
+                       (gx - ls / 2, ly, gz + ls / 2))
+    m_black = s.add_material(capi.MAT_LAMBERTIAN, (0.0, 0.0, 0.0))
+    sid = s.add_mesh(P, I, m_black, normals=N)
+    power = power_frac / (1 - power_frac) * others
+    col = np.array([1.0, 0.85, 0.6])
+    intensity = col / float(col @ lumw) * power / (ls * ls * np.pi)
+    s.add_area_light(sid, tuple(float(v) for v in intensity))
+    s.meta.update(gap_frac=gap_frac, power_frac=power_frac, gap_center=[gx, y, gz], gap_side=side,
+                  emitter_power=float(power), other_power=float(others))
     return s
 
 

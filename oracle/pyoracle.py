@@ -52,6 +52,14 @@ def lib():
         L.orc_light_sample.argtypes = [VP, C.c_int, dp, dp, C.c_size_t, dp, dp]
         L.orc_light_pmf.argtypes = [VP, dp]
         L.orc_last_error.argtypes = [C.c_char_p, C.c_size_t]
+        fp = C.POINTER(C.c_float)
+        L.orc_guide_forward.argtypes = [dp, fp, C.c_int, C.c_int, dp]
+        L.orc_guide_pdf.argtypes = [dp, dp]
+        L.orc_guide_pdf.restype = C.c_double
+        L.orc_guide_sample.argtypes = [dp, C.c_double, C.c_double, C.c_double, dp]
+        L.orc_guide_loss_grad.argtypes = [dp, fp, fp, C.c_int, C.c_int, dp]
+        L.orc_guide_loss_grad.restype = C.c_double
+        L.orc_guide_features.argtypes = [dp, dp, dp, dp, dp, fp]
         _lib = L
     return _lib
 
@@ -217,3 +225,57 @@ def phase_batch(ptype, g, din, dout, rnd):
     smp = np.zeros((n, 3))
     lib().orc_phase_batch(ptype, g, _dp(din), _dp(dout), _dp(rnd), n, _dp(val), _dp(pdf), _dp(smp))
     return val, pdf, smp
+
+
+# ------------------------------------------------------------ guiding network
+GUIDE_NUM_PARAMS = 25384
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def guide_forward(params, feats, emulate_bf16=True):
+    """Raw outputs (n, 40) of the guiding MLP for feature rows (n, 8)."""
+    p = np.ascontiguousarray(params, np.float64)
+    f = np.ascontiguousarray(feats, np.float32)
+    out = np.zeros((len(f), 40), np.float64)
+    lib().orc_guide_forward(_dp(p), _fp(f), len(f), int(emulate_bf16), _dp(out))
+    return out
+
+
+def guide_loss_grad(params, feats, dirs, emulate_bf16=False):
+    """Normalised loss -sum w log q / sum w and its gradient (FP64)."""
+    p = np.ascontiguousarray(params, np.float64)
+    f = np.ascontiguousarray(feats, np.float32)
+    d = np.ascontiguousarray(dirs, np.float32)
+    g = np.zeros(GUIDE_NUM_PARAMS, np.float64)
+    loss = lib().orc_guide_loss_grad(_dp(p), _fp(f), _fp(d), len(f), int(emulate_bf16), _dp(g))
+    return loss, g
+
+
+def guide_pdf(raw, dirs):
+    raw = np.ascontiguousarray(raw, np.float64)
+    out = np.empty(len(dirs))
+    for i, d in enumerate(np.asarray(dirs, np.float64)):
+        d = np.ascontiguousarray(d)
+        out[i] = lib().orc_guide_pdf(_dp(raw), _dp(d))
+    return out
+
+
+def guide_sample(raw, u):
+    """Directions for rows of u = (u_lobe, u1, u2)."""
+    raw = np.ascontiguousarray(raw, np.float64)
+    out = np.empty((len(u), 3))
+    d = np.zeros(3)
+    for i, (a, b, c) in enumerate(np.asarray(u, np.float64)):
+        lib().orc_guide_sample(_dp(raw), a, b, c, _dp(d))
+        out[i] = d
+    return out
+
+
+def guide_features(pos, nrm, wo, lo, hi):
+    out = np.zeros(8, np.float32)
+    arrs = [np.ascontiguousarray(v, np.float64) for v in (pos, nrm, wo, lo, hi)]
+    lib().orc_guide_features(*[_dp(a) for a in arrs], _fp(out))
+    return out

@@ -1,0 +1,165 @@
+"""Learned path guiding on the device.
+
+  * the BF16 MFMA forward pass against the oracle's BF16-emulating FP64 MLP;
+  * the device gradient (forward, backward and weight-gradient kernels)
+    against the oracle's analytic gradient with the same BF16 rounding points;
+  * the first Adam step against its closed form (m_hat = g, v_hat = g^2);
+  * training on a known target distribution lowers the loss;
+  * guided rendering is unbiased: per-pixel means match the plain tracer's
+    (z-test over independent seeds);
+  * what the trained network gains: relMSE against a high-spp plain
+    reference, guided vs plain at equal spp on the guiding workload.
+"""
+import numpy as np
+import pytest
+
+from lajolla_public_amd import api, scenes
+from oracle import pyoracle as po
+from test_oracle_guide import random_batch, random_params
+
+pytestmark = pytest.mark.gpu
+
+
+def test_forward_matches_oracle():
+    g = api.Guide(batch_size=4096)
+    p = random_params(11).astype(np.float32)
+    g.set_params(p)
+    feats, _ = random_batch(12, 1000)  # not a multiple of the 64-sample wave tile
+    dev = g.forward(feats)
+    orc = po.guide_forward(p.astype(np.float64), feats, emulate_bf16=True)
+    err = np.abs(dev - orc)
+    scale = np.abs(orc).max()
+    print(f"forward: max |err| {err.max():.3e}, 99.9% {np.quantile(err, 0.999):.3e}, scale {scale:.3f}", flush=True)
+    assert np.quantile(err, 0.999) < 2e-3 * scale
+    assert err.max() < 2e-2 * scale
+    # and the un-emulated FP64 network is within BF16 precision
+    orc64 = po.guide_forward(p.astype(np.float64), feats, emulate_bf16=False)
+    assert np.abs(dev - orc64).max() < 0.05 * scale
+
+
+def per_layer_rel(a, b):
+    out = []
+    o = 0
+    for ni, no in [(160, 64), (64, 64), (64, 64), (64, 64), (64, 40)]:
+        for sl in (slice(o, o + ni * no), slice(o + ni * no, o + ni * no + no)):
+            out.append(np.linalg.norm(a[sl] - b[sl]) / max(np.linalg.norm(b[sl]), 1e-30))
+        o += ni * no + no
+    return np.array(out)
+
+
+def test_gradient_matches_oracle():
+    g = api.Guide(batch_size=2048)
+    p = random_params(13).astype(np.float32)
+    g.set_params(p)
+    feats, dirs = random_batch(14, 777)
+    loss_d, grad_d = g.gradient(feats, dirs)
+    loss_o, grad_o = po.guide_loss_grad(p.astype(np.float64), feats, dirs, emulate_bf16=True)
+    rel = per_layer_rel(grad_d.astype(np.float64), grad_o)
+    print("per-tensor relative L2 error:", np.array2string(rel, precision=5), flush=True)
+    assert abs(loss_d - loss_o) < 1e-3 * abs(loss_o)
+    assert rel.max() < 1e-2
+    cos = grad_d @ grad_o / (np.linalg.norm(grad_d) * np.linalg.norm(grad_o))
+    assert cos > 0.9999
+    # the FP64 gradient without BF16 rounding is the same direction
+    _, grad64 = po.guide_loss_grad(p.astype(np.float64), feats, dirs, emulate_bf16=False)
+    assert grad_d @ grad64 / (np.linalg.norm(grad_d) * np.linalg.norm(grad64)) > 0.995
+
+
+def test_first_adam_step_closed_form():
+    lr, eps = 1e-3, 1e-8
+    g = api.Guide(batch_size=1024, learning_rate=lr, epsilon=eps)
+    p = random_params(15).astype(np.float32)
+    g.set_params(p)
+    feats, dirs = random_batch(16, 512)
+    _, grad = g.gradient(feats, dirs)
+    g.train_step(feats, dirs)
+    p1 = g.get_params()
+    expect = p - lr * grad / (np.abs(grad) + eps)
+    np.testing.assert_allclose(p1, expect, rtol=0, atol=2e-6)
+
+
+def test_training_fits_a_known_distribution():
+    """Targets drawn from a vMF whose mean follows the position feature: the
+    loss (cross-entropy to the target) must fall well below its start."""
+    rng = np.random.default_rng(17)
+    n = 8192
+    g = api.Guide(batch_size=n, learning_rate=5e-3, seed=3)
+
+    def batch():
+        feats = np.zeros((n, 8), np.float32)
+        feats[:, :7] = rng.uniform(0, 1, (n, 7))
+        phi = 2 * np.pi * feats[:, 0]
+        mu = np.stack([np.cos(phi), np.sin(phi), np.zeros(n)], 1) * 0.6 + [0, 0, 0.8]
+        mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+        raw = np.zeros(40)
+        d = np.zeros((n, 4), np.float32)
+        for i in range(n):
+            raw[0:3] = mu[i]
+            raw[3] = np.log(20.0)
+            raw[4] = 50.0
+            d[i, :3] = po.guide_sample(raw, rng.uniform(0, 1, (1, 3)))[0]
+        d[:, 3] = 1.0
+        return feats, d
+
+    f0, d0 = batch()
+    start = g.gradient(f0, d0)[0]
+    losses = []
+    for k in range(150):
+        if k % 30 == 0:
+            f, d = batch()
+        losses.append(g.train_step(f, d))
+    end = g.gradient(f0, d0)[0]
+    print(f"loss {start:.3f} -> {end:.3f}", flush=True)
+    # entropy of a kappa = 20 vMF is about -0.16; a uniform sphere has log(4 pi) = 2.53
+    assert end < start - 1.5
+    assert np.isfinite(losses).all()
+
+
+def gap_scene(W=64, H=36, spp=8):
+    return scenes.terrain_gap(seed=7, width=W, height=H, spp=spp, max_depth=8, grid=(200, 100), n_spheres=8,
+                              sphere_subdiv=3, n_lights=64)
+
+
+def test_guided_render_is_unbiased():
+    desc = gap_scene(spp=64)
+    sc = api.Scene(desc)
+    g = api.Guide(batch_size=1 << 16, seed=5)
+    # train first, then freeze the network for the comparison
+    opts = scenes.render_options(**dict(desc.options, samples_per_pixel=16))
+    api.render_guided(sc, g, opts, gopts=api.guide_options(train_spp=16, steps_per_pass=4))
+    K = 8
+    gi, pi = [], []
+    for k in range(K):
+        o = scenes.render_options(**dict(desc.options, seed=100 + k))
+        gi.append(api.render_guided(sc, g, o, gopts=api.guide_options(train_spp=0)))
+        pi.append(sc.render(options=o))
+    gi, pi = np.array(gi, np.float64), np.array(pi, np.float64)
+    gm, gv = gi.mean(0), gi.var(0, ddof=1) / K
+    pm, pv = pi.mean(0), pi.var(0, ddof=1) / K
+    z = np.abs(gm - pm) / np.sqrt(gv + pv + 1e-12)
+    frac = (z > 4).mean()
+    print(f"z>4 fraction {frac:.4f}; image means guided {gm.mean():.4f} plain {pm.mean():.4f}", flush=True)
+    assert np.isfinite(gi).all()
+    assert frac < 0.02
+    assert abs(gm.mean() - pm.mean()) < 0.03 * pm.mean()
+
+
+def rel_mse(img, ref):
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+def test_trained_guide_lowers_error_at_equal_spp():
+    desc = gap_scene(W=96, H=54, spp=32)
+    sc = api.Scene(desc)
+    ref = sc.render(options=scenes.render_options(**dict(desc.options, samples_per_pixel=4096, seed=999)))
+    g = api.Guide(batch_size=1 << 18, seed=5)
+    opts = scenes.render_options(**dict(desc.options, samples_per_pixel=64))
+    api.render_guided(sc, g, opts, gopts=api.guide_options(train_spp=64, steps_per_pass=8))
+    errs_p, errs_g = [], []
+    for k in range(4):
+        o = scenes.render_options(**dict(desc.options, seed=200 + k))
+        errs_p.append(rel_mse(sc.render(options=o), ref))
+        errs_g.append(rel_mse(api.render_guided(sc, g, o, gopts=api.guide_options(train_spp=0)), ref))
+    ep, eg = np.mean(errs_p), np.mean(errs_g)
+    print(f"relMSE at 32 spp: plain {ep:.5f}, guided {eg:.5f}, ratio {ep / eg:.2f}", flush=True)
+    assert eg < ep / 1.3
