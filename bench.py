@@ -34,6 +34,9 @@ def main():
     ap.add_argument("--cpu-spp", type=int, default=0, help="spp of the bounded CPU baseline sample (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-paths", type=int, default=0)
+    ap.add_argument("--no-guide", action="store_true", help="skip the guided-vs-plain equal-time comparison")
+    ap.add_argument("--guide-ref-spp", type=int, default=2048,
+                    help="spp of the plain reference image the relMSE figures are measured against")
     ap.add_argument("--mode", choices=["render", "copycal"], default="render",
                     help="copycal: one plain FP32 elementwise kernel over 1 GiB (read) + 1 GiB (write), "
                          "to scale the FETCH_SIZE / WRITE_SIZE counters")
@@ -121,6 +124,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(desc, args.cpu_spp)
+    guiding = None
+    if not args.no_guide and args.workload == "terrain_spheres":
+        del sc
+        guiding = guided_vs_plain(args.guide_ref_spp)
 
     line = {
         "metric": metric,
@@ -162,10 +169,97 @@ def main():
             "avg_launch_ms": round(ext_ms / max(1, ext_launches), 4),
         },
         "cpu_baseline": cpu,
+        "guiding": guiding,
     }
     if cpu and cpu.get("value"):
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(line), flush=True)
+
+
+def rel_mse(img, ref):
+    import numpy as np
+    return float(np.mean((img.astype(np.float64) - ref) ** 2 / (ref.astype(np.float64) ** 2 + 1e-2)))
+
+
+def guided_vs_plain(ref_spp, plain_spp=64, train_spp=8, steps_per_pass=6):
+    """relMSE at equal wall time, learned guiding vs the plain tracer, on the
+    guiding workload (terrain_gap: the large-mesh scene under a ceiling with a
+    5% gap, 90% of the light power behind it), 1920x1080, depth 8.  Each
+    guided render starts from an untrained network: its training passes and
+    Adam steps are inside the timed wall time, and their samples stay in the
+    image.  relMSE = mean((x - ref)^2 / (ref^2 + 0.01)) against a plain
+    render at ref_spp."""
+    import numpy as np
+    from lajolla_public_amd import api, scenes
+    t0 = time.time()
+    desc = scenes.terrain_gap(seed=7)
+    sc = api.Scene(desc)
+    W, H = desc.camera["width"], desc.camera["height"]
+    log = lambda m: print(f"[bench] guiding: {m}", file=sys.stderr, flush=True)
+    log(f"scene ready in {time.time() - t0:.1f}s ({desc.num_triangles} triangles)")
+
+    def opts(spp, seed):
+        return scenes.render_options(**dict(desc.options, samples_per_pixel=spp, seed=seed))
+
+    t0 = time.time()
+    ref = sc.render(options=opts(ref_spp, 12345)).astype(np.float64)
+    log(f"reference {ref_spp} spp in {time.time() - t0:.1f}s")
+    sc.render(options=opts(4, 1))  # warm-up
+
+    def plain(spp, seed):
+        t = time.perf_counter()
+        img = sc.render(options=opts(spp, seed))
+        return (time.perf_counter() - t) * 1000, rel_mse(img, ref)
+
+    def guided(spp, seed):
+        g = api.Guide(seed=seed)
+        go = api.guide_options(mix_prob=0.5, train_spp=min(train_spp, spp), steps_per_pass=steps_per_pass)
+        t = time.perf_counter()
+        img, st = api.render_guided(sc, g, opts(spp, seed), gopts=go, stats=True)
+        ms = (time.perf_counter() - t) * 1000
+        g.close()
+        return ms, rel_mse(img, ref), st
+
+    guided(16, 7)  # warm-up (first launch of the guided kernels)
+    p_ms, p_err = plain(plain_spp, 1)
+    log(f"plain {plain_spp} spp: {p_ms:.0f} ms, relMSE {p_err:.5f}")
+    curve = []
+    for spp in (8, 16, 24, 32, 48, 64):
+        ms, err, st = guided(spp, 100 + spp)
+        curve.append({"spp": spp, "ms": round(ms, 1), "relmse": round(err, 6), "train_samples": st["train_samples"],
+                      "train_steps": st["train_steps"], "train_ms": round(st["train_ms"], 1)})
+        log(f"guided {spp} spp: {ms:.0f} ms, relMSE {err:.5f}, {st['train_steps']} Adam steps on "
+            f"{st['train_samples']} samples")
+    # equal wall time: interpolate the guided relMSE at the plain render's time (log-log)
+    t = np.array([c["ms"] for c in curve])
+    e = np.array([c["relmse"] for c in curve])
+    order = np.argsort(t)
+    t, e = t[order], e[order]
+    err_eq = float(np.exp(np.interp(np.log(p_ms), np.log(t), np.log(e))))
+    # wall time at which the guided render reaches the plain render's relMSE
+    if e.min() <= p_err:
+        k = int(np.argmax(e <= p_err))
+        if k == 0:
+            t_match = float(t[0])
+        else:
+            f = (np.log(p_err) - np.log(e[k - 1])) / (np.log(e[k]) - np.log(e[k - 1]))
+            t_match = float(np.exp(np.log(t[k - 1]) + f * (np.log(t[k]) - np.log(t[k - 1]))))
+    else:
+        t_match = None
+    return {
+        "workload": "terrain_gap: terrain_spheres (2,308,746 triangles) under a ceiling with a 5% gap; a 1x1 "
+                    "emitter above the gap carries 90% of the light power; 1920x1080, depth 8",
+        "reference_spp": ref_spp,
+        "plain": {"spp": plain_spp, "ms": round(p_ms, 1), "relmse": round(p_err, 6)},
+        "guided_curve": curve,
+        "guided_relmse_at_equal_time": round(err_eq, 6),
+        "relmse_ratio_plain_over_guided_equal_time": round(p_err / err_eq, 3),
+        "guided_time_to_plain_relmse_ms": round(t_match, 1) if t_match else None,
+        "time_ratio_guided_over_plain": round(t_match / p_ms, 3) if t_match else None,
+        "mix_prob": 0.5,
+        "train": {"train_spp": train_spp, "steps_per_pass": steps_per_pass, "batch": 1 << 20, "optimizer": "adam",
+                  "lr": 1e-3},
+    }
 
 
 def measured_traffic(workload):

@@ -23,6 +23,13 @@ __host__ __device__ inline int act_rows(int which) { return which == 8 ? kOutRow
 inline size_t act_elems(int groups) { return act_offset(8, groups) + size_t(groups) * kOutRows * 32; }
 
 __device__ __forceinline__ uint16_t bf16_bits(float v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
+// Bits of element e of a BF16 vector, through its 32-bit words (a direct
+// bit_cast of the extracted __bf16 element was seen to return element e & ~3).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint16_t bf16_elem_bits(bf16x8 v, int e) {
+    u32x4 u = __builtin_bit_cast(u32x4, v);
+    return uint16_t(u[e >> 1] >> (16 * (e & 1)));
+}
 
 __global__ void k_pack(const float *master, const int *fmap, __bf16 *ffrag, int nf, const int *bmap, __bf16 *bfrag,
                        int nb, const int *biasmap, float *fbias, int nbias) {
@@ -126,7 +133,7 @@ __global__ __launch_bounds__(512) void k_fwdbwd(const float4 *feats, const float
                     for (int kb = 0; kb < 2; kb++)
 #pragma unroll
                         for (int e = 0; e < 8; e++)
-                            dst[kslot_hidden(kb, g, e) * 32 + 16 * s + j] = __builtin_bit_cast(uint16_t, H[k][s][kb][e]);
+                            dst[kslot_hidden(kb, g, e) * 32 + 16 * s + j] = bf16_elem_bits(H[k][s][kb], e);
             }
             f32x4 gr[2][3];
 #pragma unroll

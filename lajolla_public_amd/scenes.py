@@ -456,21 +456,21 @@ def terrain_spheres(seed=7, width=1920, height=1080, spp=64, max_depth=8, grid=(
 
 def terrain_gap(seed=7, width=1920, height=1080, spp=64, max_depth=8, gap_frac=0.05, power_frac=0.9, **kw):
     """Guiding workload: the terrain_spheres scene under an occluding ceiling
-    (y = 10.5, covering the terrain and the camera) with a square gap of
-    gap_frac of its area, and a small emitter above the gap that carries
-    power_frac of the scene's total light power (lajolla's light power:
-    pi * area * luminance for area lights, pi r^2 * luminance for the
-    environment, src/lights/diffuse_area_light.inl:1-3, src/lights/envmap.inl:2-4).
-    Most of the scene sees the bright emitter only through the gap, i.e. by
-    indirect light from the patch it illuminates."""
+    plane (y = 10.5, covering the terrain) with a square gap of gap_frac of
+    its area, and a small emitter high above the gap that carries power_frac
+    of the scene's total light power (lajolla's light power: pi * area *
+    luminance for area lights, pi r^2 * luminance for the environment,
+    src/lights/diffuse_area_light.inl:1-3, src/lights/envmap.inl:2-4).  The
+    emitter lights a patch of the terrain through the gap; the rest of the
+    scene sees it only by indirect light from that patch."""
     s = terrain_spheres(seed=seed, width=width, height=height, spp=spp, max_depth=max_depth, **kw)
     s.name = "terrain_gap"
     rng = np.random.default_rng(seed + 1000)
     y = 10.5
-    x0, x1, z0, z1 = -24.0, 24.0, -14.0, 30.0
+    x0, x1, z0, z1 = -22.0, 22.0, -12.0, 12.0
     area = (x1 - x0) * (z1 - z0)
     side = math.sqrt(gap_frac * area)
-    gx, gz = float(rng.uniform(-8, 8)), float(rng.uniform(-5, 3))
+    gx, gz = float(rng.uniform(-8, 8)), float(rng.uniform(-4, 4))
     hx0, hx1, hz0, hz1 = gx - side / 2, gx + side / 2, gz - side / 2, gz + side / 2
     m_ceil = s.add_material(capi.MAT_LAMBERTIAN, (0.5, 0.5, 0.5))
     # four quads around the hole, normals facing down
@@ -493,9 +493,9 @@ def terrain_gap(seed=7, width=1920, height=1080, spp=64, max_depth=8, gap_frac=0
             others += np.pi * a * float(np.asarray(l["intensity"]) @ lumw)
         else:
             others += np.pi * r * r * float(l["env"].reshape(3) @ lumw) * l["scale"]
-    # the bright emitter: a 1 x 1 quad 2.5 above the gap, facing down
+    # the bright emitter: a 1 x 1 quad 25 above the gap, facing down
     ls = 1.0
-    ly = y + 2.5
+    ly = y + 25.0
     P, I, N = quad((gx - ls / 2, ly, gz - ls / 2), (gx - ls / 2, ly, gz + ls / 2), (gx + ls / 2, ly, gz + ls / 2),
                    (gx + ls / 2, ly, gz - ls / 2))
     if N[0][1] > 0:
