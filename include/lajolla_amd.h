@@ -424,6 +424,45 @@ int ljx_render_guided_device(void *scene, void *guide, const ljx_render_options 
                              const ljx_exec_options *exec, const ljx_guide_options *gopts,
                              float *rgb_out_device, ljx_render_stats *stats);
 
+/* ---- batched rendering ---------------------------------------------------
+ * Many render jobs of uneven size in one captured HIP graph: the jobs of a
+ * scene share one wavefront loop whose passes are packed with path slots
+ * from several jobs, so small images still fill the device.  A job's image
+ * equals ljx_render_device of the same camera, spp and seed up to the order
+ * of its per-pixel sums.  No reference counterpart: lajolla renders one
+ * scene per call (src/main.cpp:33-45). */
+typedef struct ljx_batch_job {
+    void *scene;                   /* scene handle: geometry, materials, lights, integrator */
+    ljx_camera_desc camera;        /* this job's camera and resolution */
+    int32_t samples_per_pixel;
+    int32_t max_depth;             /* >= 0; uniform across the jobs of one scene */
+    uint64_t seed;
+    float *rgb_out_device;         /* height * width * 3 floats on the scene's device */
+} ljx_batch_job;
+
+typedef struct ljx_batch_stats {
+    double total_ms;               /* wall time of the call */
+    double build_ms;               /* graph capture + instantiation (+ autotuning) */
+    double launch_ms;              /* HIP-event time of the graph launch */
+    uint64_t samples;              /* camera paths of all jobs */
+    uint64_t passes;
+    uint64_t graph_nodes;
+    int32_t num_scenes;
+    int32_t pad;
+} ljx_batch_stats;
+
+/* Renders every job; per_job_ms (may be NULL, njobs entries) gets each job's
+ * completion time from the start of the launch.  exec may be NULL: each
+ * scene then uses its autotuned settings (ljx_autotune, run on first use). */
+int ljx_render_batch_device(const ljx_batch_job *jobs, size_t njobs, const ljx_exec_options *exec,
+                            double *per_job_ms, ljx_batch_stats *stats);
+
+/* Times candidate launch settings (paths in flight, shading block size, sort
+ * threshold) on short renders of the scene with opts (NULL: scene options)
+ * and keeps the fastest for the scene's batched renders; best (may be NULL)
+ * receives it.  Results do not depend on these settings. */
+int ljx_autotune(void *scene, const ljx_render_options *opts, ljx_exec_options *best);
+
 /* Device memory helpers for callers without their own allocator. */
 int ljx_device_alloc(int device, size_t bytes, void **ptr);
 int ljx_device_free(void *ptr);

@@ -243,6 +243,30 @@ class GuideOptions(C.Structure):
     ]
 
 
+class BatchJob(C.Structure):
+    _fields_ = [
+        ("scene", C.c_void_p),
+        ("camera", CameraDesc),
+        ("samples_per_pixel", C.c_int32),
+        ("max_depth", C.c_int32),
+        ("seed", C.c_uint64),
+        ("rgb_out_device", C.c_void_p),
+    ]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("total_ms", C.c_double),
+        ("build_ms", C.c_double),
+        ("launch_ms", C.c_double),
+        ("samples", C.c_uint64),
+        ("passes", C.c_uint64),
+        ("graph_nodes", C.c_uint64),
+        ("num_scenes", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+
 # Every symbol include/lajolla_amd.h declares, with its ctypes signature.
 VP = C.c_void_p
 SIGNATURES = {
@@ -272,6 +296,9 @@ SIGNATURES = {
     "ljx_guide_train_step_device": ([VP, VP, VP, C.c_size_t, C.POINTER(C.c_double)], C.c_int),
     "ljx_render_guided_device": ([VP, VP, C.POINTER(RenderOptions), C.POINTER(ExecOptions),
                                   C.POINTER(GuideOptions), VP, C.POINTER(RenderStats)], C.c_int),
+    "ljx_render_batch_device": ([C.POINTER(BatchJob), C.c_size_t, C.POINTER(ExecOptions), C.POINTER(C.c_double),
+                                 C.POINTER(BatchStats)], C.c_int),
+    "ljx_autotune": ([VP, C.POINTER(RenderOptions), C.POINTER(ExecOptions)], C.c_int),
     "ljx_device_alloc": ([C.c_int, C.c_size_t, C.POINTER(VP)], C.c_int),
     "ljx_device_free": ([VP], C.c_int),
     "ljx_copy_to_device": ([VP, VP, C.c_size_t], C.c_int),

@@ -25,6 +25,7 @@ class Scene:
         h = C.c_void_p()
         capi.check(self.lib.ljx_scene_create(C.byref(d), device, C.byref(h)), self.lib)
         self.h = h
+        self.device = device
         self.width = desc.camera["width"]
         self.height = desc.camera["height"]
 
@@ -228,3 +229,40 @@ def render_guided(scene: Scene, guide: Guide, options=None, exec_options=None, g
     if stats:
         return img, stats_dict(st)
     return img
+
+
+def render_batch(jobs, exec_options=None):
+    """Batched render: jobs is a list of dicts with scene (Scene), camera
+    (camera dict), spp, max_depth, seed.  Returns (images, per-job completion
+    ms from the launch start, stats dict)."""
+    from .scenes import camera_desc
+    lib = capi.load()
+    n = len(jobs)
+    arr = (capi.BatchJob * n)()
+    outs = []
+    for i, j in enumerate(jobs):
+        cam = j["camera"]
+        buf = DeviceBuffer(cam["width"] * cam["height"] * 3 * 4, j["scene"].device)
+        outs.append(buf)
+        a = arr[i]
+        a.scene = j["scene"].h
+        a.camera = camera_desc(cam)
+        a.samples_per_pixel = j["spp"]
+        a.max_depth = j.get("max_depth", 8)
+        a.seed = j.get("seed", capi.DEFAULT_SEED)
+        a.rgb_out_device = outs[-1].ptr
+    ms = (C.c_double * n)()
+    st = capi.BatchStats()
+    capi.check(lib.ljx_render_batch_device(arr, n, C.byref(exec_options) if exec_options is not None else None, ms,
+                                           C.byref(st)), lib)
+    imgs = [b.download(np.zeros((j["camera"]["height"], j["camera"]["width"], 3), np.float32))
+            for b, j in zip(outs, jobs)]
+    return imgs, list(ms), {k: getattr(st, k) for k, _ in capi.BatchStats._fields_}
+
+
+def autotune(scene: Scene, options=None):
+    """Runs the launch-settings autotuner for a scene; returns the chosen ExecOptions."""
+    best = capi.ExecOptions()
+    capi.check(scene.lib.ljx_autotune(scene.h, C.byref(options) if options is not None else None, C.byref(best)),
+               scene.lib)
+    return best

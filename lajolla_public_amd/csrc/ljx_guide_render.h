@@ -19,10 +19,7 @@ namespace ljx {
 // Per-vertex guide stream: independent of the path's own PCG stream, so the
 // BSDF draws of a guided path are the ones the unguided tracer makes.
 __device__ __forceinline__ void guide_draws(const PathBufs &P, uint32_t pid, int b, float u[4]) {
-    int s, x, y;
-    bool valid;
-    slot_pixel(P, int(pid), s, x, y, valid);
-    uint64_t gs = path_stream(P, s, x, y) ^ 0x2545f4914f6cdd1dULL;
+    uint64_t gs = slot_info(P, int(pid)).stream() ^ 0x2545f4914f6cdd1dULL;
     uint64_t inc = pcg_inc(gs);
     uint64_t st = pcg_init_state(gs, P.seed + 0x9e3779b97f4a7c15ULL * uint64_t(b + 1));
 #pragma unroll

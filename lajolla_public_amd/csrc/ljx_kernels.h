@@ -27,12 +27,25 @@ namespace ljx {
 constexpr int kNumMatTypes = 9;
 constexpr int kBucketMiss = 9;
 constexpr int kBucketNoMat = 10;
-constexpr int kNumBuckets = 11;
+constexpr int kBucketAny = 11;  // unsorted hits (below the sort threshold), shaded by runtime dispatch
+constexpr int kNumBuckets = 12;
 // counter slots
 constexpr int C_EXT = 0;      // C_EXT + (0|1): extension queues
 constexpr int C_SH = 2;       // shadow queue
 constexpr int C_BUCKET = 4;   // C_BUCKET + bucket
-constexpr int C_TOTAL = 16;
+constexpr int C_TOTAL = 16 + 4;
+
+// One job of a batched render: its camera, image and place in the batch's
+// global work index space (sample-major over its 8x8-tiled pixel grid).
+struct BatchJob {
+    CameraRec camera;
+    long long work_base;
+    int W, H, ntx8, npix8;
+    int spp, pad;
+    uint64_t seed;
+    double *accum;    // W * H * 3
+    float *out;       // W * H * 3
+};
 // 64-bit statistics slots
 constexpr int S_EXT_RAYS = 0, S_SH_RAYS = 1, S_NODES = 2, S_PRIMS = 3, S_SH_NODES = 4, S_SH_PRIMS = 5,
               S_NUM = 8;
@@ -78,6 +91,11 @@ struct PathBufs {
     uint32_t guide_mask;  // material types sampled from the mixture
     float guide_mix;   // probability of the guide in the mixture
     float box_lo[3], box_scale[3];  // position features: (p - lo) * scale
+    // batched rendering: slot i of a pass holds global work item work0 + i
+    const BatchJob *jobs;  // null for a single render
+    int njobs;
+    int sort_threshold;    // bounces with fewer extension rays skip the material sort
+    long long work0;
 };
 
 // Shadow-queue entries carry the path slot in the low 24 bits and the vertex
@@ -87,16 +105,45 @@ constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1;
 // Material types never guided: their lobes are near-specular transmission.
 __host__ __device__ constexpr bool guidable(int mt) { return mt != 2 && mt != 5; }
 
-__device__ __forceinline__ void slot_pixel(const PathBufs &P, int i, int &s, int &x, int &y, bool &valid) {
-    s = P.pass_s0 + i / P.pass_nq;
-    int q = P.pass_q0 + i % P.pass_nq;
+// The camera path held by slot i: its job (0 for a single render), sample
+// index, pixel, and the PCG stream id s * W * H + y * W + x of its image.
+struct SlotInfo {
+    int job, s, x, y, W, H;
+    bool valid;
+    __device__ uint64_t stream() const { return uint64_t(s) * uint64_t(W) * uint64_t(H) + uint64_t(y) * W + x; }
+};
+__device__ __forceinline__ SlotInfo slot_info(const PathBufs &P, int i) {
+    SlotInfo r;
+    int q, ntx8;
+    if (P.jobs) {
+        long long w = P.work0 + i;
+        int lo = 0, hi = P.njobs - 1;  // last job with work_base <= w
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (P.jobs[mid].work_base <= w) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchJob &J = P.jobs[lo];
+        long long local = w - J.work_base;
+        r.job = lo;
+        r.s = int(local / J.npix8);
+        q = int(local % J.npix8);
+        ntx8 = J.ntx8;
+        r.W = J.W;
+        r.H = J.H;
+    } else {
+        r.job = 0;
+        r.s = P.pass_s0 + i / P.pass_nq;
+        q = P.pass_q0 + i % P.pass_nq;
+        ntx8 = P.ntx8;
+        r.W = P.W;
+        r.H = P.H;
+    }
     int tile = q >> 6, w = q & 63;
-    x = (tile % P.ntx8) * 8 + (w & 7);
-    y = (tile / P.ntx8) * 8 + (w >> 3);
-    valid = x < P.W && y < P.H;
-}
-__device__ __forceinline__ uint64_t path_stream(const PathBufs &P, int s, int x, int y) {
-    return uint64_t(s) * uint64_t(P.W) * uint64_t(P.H) + uint64_t(y) * P.W + x;
+    r.x = (tile % ntx8) * 8 + (w & 7);
+    r.y = (tile / ntx8) * 8 + (w >> 3);
+    r.valid = r.x < r.W && r.y < r.H;
+    return r;
 }
 
 // Counter resets folded into kernels that do not read those counters.
@@ -119,16 +166,20 @@ __global__ void k_raygen(DeviceScene S, PathBufs P, int compact) {
     if (!compact && blockIdx.x == 0 && threadIdx.x == 0) P.ctr[C_EXT + 0] = n;
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         int i = base + threadIdx.x;
-        int s = 0, x = 0, y = 0;
+        SlotInfo si{};
         bool valid = false;
-        if (i < n) slot_pixel(P, i, s, x, y, valid);
+        if (i < n) {
+            si = slot_info(P, i);
+            valid = si.valid;
+        }
         if (valid) {
-            uint64_t stream = path_stream(P, s, x, y);
+            uint64_t stream = si.stream();
             uint64_t inc = pcg_inc(stream);
-            uint64_t st = pcg_init_state(stream, P.seed);
+            uint64_t st = pcg_init_state(stream, P.jobs ? P.jobs[si.job].seed : P.seed);
             double r0 = pcg_double(st, inc), r1 = pcg_double(st, inc);
             D3 o, d;
-            sample_primary(S.camera, (x + r0) / P.W, (y + r1) / P.H, o, d);
+            const CameraRec &cam = P.jobs ? P.jobs[si.job].camera : S.camera;
+            sample_primary(cam, (si.x + r0) / si.W, (si.y + r1) / si.H, o, d);
             P.rng[i] = st;
             P.ray_o[i] = make_float4(float(o.x), float(o.y), float(o.z), 0.0f);
             P.ray_d[i] = make_float4(float(d.x), float(d.y), float(d.z), __builtin_huge_valf());
@@ -152,6 +203,7 @@ __global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int c
     __shared__ int stack[kStackLds * 256];
     const uint32_t *q = cur ? P.extq1 : P.extq0;
     int n = P.ctr[C_EXT + cur];
+    const bool sorted = n >= P.sort_threshold;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         P.ctr[C_EXT + (cur ^ 1)] = 0;
         P.ctr[C_SH] = 0;
@@ -176,7 +228,7 @@ __global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int c
                 P.hit[pid] = make_float4(h.t, h.u, h.v, __int_as_float(h.slot));
                 int gid = __float_as_int(S.prims[3 * h.slot].w);
                 int mid = S.shapes[S.prim_shape[gid]].material_id;
-                bucket = mid >= 0 ? S.materials[mid].type : kBucketNoMat;
+                bucket = mid < 0 ? kBucketNoMat : (sorted ? int(S.materials[mid].type) : kBucketAny);
             } else {
                 P.hit[pid] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
                 bucket = kBucketMiss;
@@ -211,14 +263,7 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
     D3 org = f3d(ro), dir = f3d(rd);
     int b = P.depth[pid];
     uint64_t st = P.rng[pid];
-    uint64_t inc = 0;
-    {
-        // recover the stream of this slot
-        int s, x, y;
-        bool valid;
-        slot_pixel(P, int(pid), s, x, y, valid);
-        inc = pcg_inc(path_stream(P, s, x, y));
-    }
+    const uint64_t inc = pcg_inc(slot_info(P, int(pid)).stream());
     float4 tp4 = P.tp[pid];
     D3 tp{tp4.x, tp4.y, tp4.z};
     double eta_scale = tp4.w;
@@ -234,7 +279,7 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
     float4 gd = make_float4(0.f, 0.f, 0.f, -1.f);
     const float4 *lobes = nullptr;
     const double alpha = GUIDE ? double(P.guide_mix) : 0.0;
-    if (GUIDE && guidable(MT)) {
+    if (GUIDE && guidable(MT >= 0 ? MT : int(mat.type))) {
         gd = P.gdir[pid];
         guided = gd.w >= 0.f;
         lobes = P.glob + size_t(pid) * guide::kLobes;
@@ -365,10 +410,12 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
     P.rad[pid] = make_float4(float(rad.x), float(rad.y), float(rad.z), 0.f);
 }
 
+// MT = -1: the unsorted bucket, with the material type dispatched per path.
 template <int MT, bool SPH, bool GUIDE>
 __global__ __launch_bounds__(256) void k_shade(DeviceScene S, PathBufs P, int cur) {
-    int n = P.ctr[C_BUCKET + MT];
-    const uint32_t *q = P.bucket + size_t(MT) * P.capacity;
+    constexpr int B = MT >= 0 ? MT : kBucketAny;
+    int n = P.ctr[C_BUCKET + B];
+    const uint32_t *q = P.bucket + size_t(B) * P.capacity;
     uint32_t *next = cur ? P.extq0 : P.extq1;
     int stride = gridDim.x * blockDim.x;
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -474,10 +521,9 @@ __global__ void k_accumulate(PathBufs P) {
     int nq = P.pass_nq;
     int stride = gridDim.x * blockDim.x;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nq; j += stride) {
-        int s, x, y;
-        bool valid;
-        slot_pixel(P, j, s, x, y, valid);
-        if (!valid) continue;
+        SlotInfo si = slot_info(P, j);
+        if (!si.valid) continue;
+        const int x = si.x, y = si.y;
         double r = 0, g = 0, b = 0;
         for (int k = 0; k < P.pass_ns; k++) {
             float4 v = P.rad[k * nq + j];
@@ -491,6 +537,36 @@ __global__ void k_accumulate(PathBufs P) {
         P.accum[o + 1] += g;
         P.accum[o + 2] += b;
     }
+}
+
+// Batched renders: per-slot atomic sums into each job's image.
+__global__ void k_accumulate_batch(PathBufs P, int n) {
+    int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        SlotInfo si = slot_info(P, i);
+        if (!si.valid) continue;
+        float4 v = P.rad[i];
+        if (P.volpath && !(isfinite(v.x) && isfinite(v.y) && isfinite(v.z))) continue;
+        double *a = P.jobs[si.job].accum + (size_t(si.y) * si.W + si.x) * 3;
+        atomicAdd(a, double(v.x));
+        atomicAdd(a + 1, double(v.y));
+        atomicAdd(a + 2, double(v.z));
+    }
+}
+
+__global__ void k_job_clear(const BatchJob *jobs, int j) {
+    const BatchJob &J = jobs[j];
+    size_t n = size_t(J.W) * J.H * 3;
+    size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) J.accum[i] = 0.0;
+}
+
+__global__ void k_job_resolve(const BatchJob *jobs, int j) {
+    const BatchJob &J = jobs[j];
+    size_t n = size_t(J.W) * J.H * 3;
+    double inv = 1.0 / double(J.spp);
+    size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) J.out[i] = float(J.accum[i] * inv);
 }
 
 __global__ void k_resolve(const double *accum, float *out, size_t n, double inv_spp) {

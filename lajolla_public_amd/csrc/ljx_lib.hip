@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -125,6 +126,7 @@ struct Scene {
     bool has_spheres = false;
     unsigned mat_types_present = 0;
     bool has_media = false;
+    int num_media = 0;
     double bound_center[3] = {0, 0, 0};
     double bound_radius = 0;
     float box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};  // axis-aligned bounds of every primitive
@@ -132,6 +134,8 @@ struct Scene {
     double build_ms = 0;
     std::unique_ptr<RenderCtx> rc;
     int rc_alloc_W = 0, rc_alloc_H = 0;
+    bool tuned = false;
+    ljx_exec_options tuned_exec{};  // autotuned launch settings
 
     ~Scene() {
         if (device >= 0) hipSetDevice(device);
@@ -166,6 +170,29 @@ struct Scene {
         return static_cast<T *>(p);
     }
 };
+
+// Camera record with lajolla's sample-to-camera matrix (src/camera.cpp:7-21).
+CameraRec make_camera(const ljx_camera_desc &c) {
+    if (c.width <= 0 || c.height <= 0) fail(LJX_ERR_INVALID_ARGUMENT, "camera width/height must be positive");
+    if (c.filter_type > 2) fail(LJX_ERR_INVALID_ARGUMENT, "unknown filter type");
+    CameraRec cam;
+    std::memcpy(cam.cam_to_world, c.cam_to_world, sizeof(cam.cam_to_world));
+    cam.width = c.width;
+    cam.height = c.height;
+    cam.filter_type = int(c.filter_type);
+    cam.filter_param = c.filter_param;
+    cam.medium_id = c.medium_id;
+    double aspect = double(cam.width) / double(cam.height);
+    double sm[16] = {-0.5, 0, 0, 0, 0, -0.5 * aspect, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    double tm[16] = {1, 0, 0, -1.0, 0, 1, 0, -1.0 / aspect, 0, 0, 1, 0, 0, 0, 0, 1};
+    double cot = 1.0 / std::tan((kPi / 180.0) * (c.fov / 2.0));
+    double pm[16] = {cot, 0, 0, 0, 0, cot, 0, 0, 0, 0, 1, -1, 0, 0, 1, 0};
+    double st[16], c2s[16];
+    mat_mul4(sm, tm, st);
+    mat_mul4(st, pm, c2s);
+    if (!mat_inv4(c2s, cam.sample_to_cam)) fail(LJX_ERR_INVALID_ARGUMENT, "camera projection is singular");
+    return cam;
+}
 
 void validate(const ljx_scene_desc *d) {
     if (!d) fail(LJX_ERR_INVALID_ARGUMENT, "scene descriptor is NULL");
@@ -233,6 +260,7 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
     sc.width = d->camera.width;
     sc.height = d->camera.height;
     sc.has_media = d->num_media > 0;
+    sc.num_media = d->num_media;
 
     // ---- flatten geometry
     std::vector<float> vpos, vnrm, vuv;
@@ -440,24 +468,7 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
     }
 
     // ---- camera (src/camera.cpp:7-21)
-    CameraRec &cam = S.camera;
-    std::memcpy(cam.cam_to_world, d->camera.cam_to_world, sizeof(cam.cam_to_world));
-    cam.width = d->camera.width;
-    cam.height = d->camera.height;
-    cam.filter_type = int(d->camera.filter_type);
-    cam.filter_param = d->camera.filter_param;
-    cam.medium_id = d->camera.medium_id;
-    {
-        double aspect = double(cam.width) / double(cam.height);
-        double sm[16] = {-0.5, 0, 0, 0, 0, -0.5 * aspect, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-        double tm[16] = {1, 0, 0, -1.0, 0, 1, 0, -1.0 / aspect, 0, 0, 1, 0, 0, 0, 0, 1};
-        double cot = 1.0 / std::tan((kPi / 180.0) * (d->camera.fov / 2.0));
-        double pm[16] = {cot, 0, 0, 0, 0, cot, 0, 0, 0, 0, 1, -1, 0, 0, 1, 0};
-        double st[16], c2s[16];
-        mat_mul4(sm, tm, st);
-        mat_mul4(st, pm, c2s);
-        if (!mat_inv4(c2s, cam.sample_to_cam)) fail(LJX_ERR_INVALID_ARGUMENT, "camera projection is singular");
-    }
+    S.camera = make_camera(d->camera);
 
     // ---- upload
     S.nodes = sc.upload(bvh.nodes);
@@ -672,27 +683,28 @@ void launch(K kernel, int grid, int block, hipStream_t s, A... args) {
 }
 
 template <bool SPH, bool GUIDE>
-void shade_all_t(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s) {
+void shade_all_t(Scene &sc, PathBufs &P, int cur, int grid, int block, hipStream_t s) {
     const DeviceScene &S = sc.ds;
     unsigned m = sc.mat_types_present;
-    if (m & (1u << 0)) launch(k_shade<0, SPH, GUIDE>, grid, 256, s, S, P, cur);
-    if (m & (1u << 1)) launch(k_shade<1, SPH, GUIDE>, grid, 256, s, S, P, cur);
-    if (m & (1u << 2)) launch(k_shade<2, SPH, GUIDE>, grid, 256, s, S, P, cur);
-    if (m & (1u << 3)) launch(k_shade<3, SPH, GUIDE>, grid, 256, s, S, P, cur);
-    if (m & (1u << 4)) launch(k_shade<4, SPH, GUIDE>, grid, 256, s, S, P, cur);
-    if (m & (1u << 5)) launch(k_shade<5, SPH, GUIDE>, grid, 256, s, S, P, cur);
-    if (m & (1u << 6)) launch(k_shade<6, SPH, GUIDE>, grid, 256, s, S, P, cur);
-    if (m & (1u << 7)) launch(k_shade<7, SPH, GUIDE>, grid, 256, s, S, P, cur);
-    if (m & (1u << 8)) launch(k_shade<8, SPH, GUIDE>, grid, 256, s, S, P, cur);
-    launch(k_miss, grid, 256, s, S, P);
+    if (m & (1u << 0)) launch(k_shade<0, SPH, GUIDE>, grid, block, s, S, P, cur);
+    if (m & (1u << 1)) launch(k_shade<1, SPH, GUIDE>, grid, block, s, S, P, cur);
+    if (m & (1u << 2)) launch(k_shade<2, SPH, GUIDE>, grid, block, s, S, P, cur);
+    if (m & (1u << 3)) launch(k_shade<3, SPH, GUIDE>, grid, block, s, S, P, cur);
+    if (m & (1u << 4)) launch(k_shade<4, SPH, GUIDE>, grid, block, s, S, P, cur);
+    if (m & (1u << 5)) launch(k_shade<5, SPH, GUIDE>, grid, block, s, S, P, cur);
+    if (m & (1u << 6)) launch(k_shade<6, SPH, GUIDE>, grid, block, s, S, P, cur);
+    if (m & (1u << 7)) launch(k_shade<7, SPH, GUIDE>, grid, block, s, S, P, cur);
+    if (m & (1u << 8)) launch(k_shade<8, SPH, GUIDE>, grid, block, s, S, P, cur);
+    if (P.sort_threshold > 0) launch(k_shade<-1, SPH, GUIDE>, grid, block, s, S, P, cur);
+    launch(k_miss, grid, block, s, S, P);
 }
-void shade_all(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s, bool guide) {
+void shade_all(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s, bool guide, int block = 256) {
     if (guide) {
-        if (sc.has_spheres) shade_all_t<true, true>(sc, P, cur, grid, s);
-        else shade_all_t<false, true>(sc, P, cur, grid, s);
+        if (sc.has_spheres) shade_all_t<true, true>(sc, P, cur, grid, block, s);
+        else shade_all_t<false, true>(sc, P, cur, grid, block, s);
     } else {
-        if (sc.has_spheres) shade_all_t<true, false>(sc, P, cur, grid, s);
-        else shade_all_t<false, false>(sc, P, cur, grid, s);
+        if (sc.has_spheres) shade_all_t<true, false>(sc, P, cur, grid, block, s);
+        else shade_all_t<false, false>(sc, P, cur, grid, block, s);
     }
 }
 template <bool STATS, bool SPH>
@@ -772,6 +784,8 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
     }
     ljx_exec_options e{};
     if (ex) e = *ex;
+    const int shade_block = e.block_size > 0 ? e.block_size : 256;
+    if (shade_block % 64 || shade_block > 256) fail(LJX_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
     HIPX(hipSetDevice(sc.device));
     hipStream_t s = sc.stream;
     int W = sc.width, H = sc.height;
@@ -803,6 +817,10 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
     P.volpath = vol ? 1 : 0;
     P.max_null = opt.max_null_collisions;
     P.rec_depth = 0;
+    P.jobs = nullptr;
+    P.njobs = 0;
+    P.work0 = 0;
+    P.sort_threshold = e.sort_threshold < 0 ? INT_MAX : e.sort_threshold;
     GuideCtx *gc = nullptr;
     if (guided) {
         ensure_guide(sc, rc, gopt.max_record_depth);
@@ -824,7 +842,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
         }
     }
     const DeviceScene &S = sc.ds;
-    int shade_grid = std::max(1, std::min(capacity / 256 + 1, sc.num_cu * 16));
+    int shade_grid = std::max(1, std::min(capacity / shade_block + 1, sc.num_cu * 16 * 256 / shade_block));
     int guide_grid = std::max(1, std::min(capacity / 256 + 1, sc.num_cu * 2));
     int tgrid = rc.trace_grid;
     Timer tm{e.collect_timing != 0, s};
@@ -880,7 +898,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
                 }
                 tm.begin(2, ev);
                 if (vol) vol_shade_all(S, P, cur, shade_grid, s, sc.has_spheres);
-                else shade_all(sc, P, cur, shade_grid, s, guided);
+                else shade_all(sc, P, cur, shade_grid, s, guided, shade_block);
                 tm.end(2, ev);
                 tm.begin(1, ev);
                 if (vol) vol_shadow(S, P, tgrid, s, stats_on, sc.has_spheres);
@@ -946,6 +964,281 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
         stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     } else {
         for (int k = 0; k < 6; k++) tm.total(k);
+    }
+}
+
+// ------------------------------------------------------------- autotuner
+// Times short renders of the scene's own camera (about 8M camera paths each)
+// for candidate settings and keeps the fastest: paths in flight x shading
+// block size, then the sort threshold at the best pair.
+ljx_exec_options autotune_impl(Scene &sc, const ljx_render_options *o) {
+    ljx_render_options opt = o ? *o : sc.options;
+    if (opt.max_depth < 0) opt.max_depth = 8;
+    const long long npix = (long long)sc.width * sc.height;
+    opt.samples_per_pixel = int(std::max<long long>(1, std::min<long long>(1 << 12, (8ll << 20) / npix)));
+    float *out = nullptr;
+    HIPX(hipMalloc(&out, size_t(npix) * 3 * sizeof(float)));
+    struct Free {
+        void *p;
+        ~Free() { hipFree(p); }
+    } f{out};
+    auto trial = [&](const ljx_exec_options &e) {
+        ljx_render_stats st;
+        render_impl(sc, &opt, &e, out, &st);  // warm (allocation, first launch)
+        render_impl(sc, &opt, &e, out, &st);
+        return st.total_ms;
+    };
+    ljx_exec_options best{};
+    double best_ms = 1e300;
+    for (int cap : {1 << 20, 1 << 21, 1 << 22, 1 << 23})
+        for (int blk : {128, 256}) {
+            ljx_exec_options e{};
+            e.max_paths_in_flight = cap;
+            e.block_size = blk;
+            double ms = trial(e);
+            if (ms < best_ms) best_ms = ms, best = e;
+        }
+    for (int thr : {4096, 65536}) {
+        ljx_exec_options e = best;
+        e.sort_threshold = thr;
+        double ms = trial(e);
+        if (ms < best_ms) best_ms = ms, best = e;
+    }
+    sc.tuned = true;
+    sc.tuned_exec = best;
+    return best;
+}
+
+// ------------------------------------------------------------ batch render
+struct JobEvents {
+    std::vector<hipEvent_t> ev;
+    ~JobEvents() {
+        for (auto e : ev)
+            if (e) hipEventDestroy(e);
+    }
+};
+
+void batch_impl(const ljx_batch_job *jobs, size_t njobs, const ljx_exec_options *ex, double *per_job_ms,
+                ljx_batch_stats *bst) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (njobs == 0) {
+        if (bst) std::memset(bst, 0, sizeof(*bst));
+        return;
+    }
+    if (!jobs) fail(LJX_ERR_INVALID_ARGUMENT, "jobs is NULL");
+    // group by scene, keeping first-appearance order
+    std::vector<std::pair<Scene *, std::vector<int>>> groups;
+    for (size_t i = 0; i < njobs; i++) {
+        const ljx_batch_job &J = jobs[i];
+        Scene *sc = as_scene(J.scene);
+        if (J.samples_per_pixel <= 0) fail(LJX_ERR_INVALID_ARGUMENT, "job " + std::to_string(i) + ": spp must be positive");
+        if (J.max_depth < 0) fail(LJX_ERR_INVALID_ARGUMENT, "job " + std::to_string(i) + ": max_depth must be >= 0");
+        if (!J.rgb_out_device) fail(LJX_ERR_INVALID_ARGUMENT, "job " + std::to_string(i) + ": output is NULL");
+        if (J.camera.medium_id < -1 || J.camera.medium_id >= sc->num_media)
+            fail(LJX_ERR_INVALID_ARGUMENT, "job " + std::to_string(i) + ": camera medium id out of range");
+        make_camera(J.camera);  // validates
+        auto it = std::find_if(groups.begin(), groups.end(), [&](auto &g) { return g.first == sc; });
+        if (it == groups.end()) groups.push_back({sc, {int(i)}});
+        else it->second.push_back(int(i));
+    }
+    const int dev = groups[0].first->device;
+    for (auto &g : groups) {
+        if (g.first->device != dev) fail(LJX_ERR_INVALID_ARGUMENT, "all scenes of a batch must be on one device");
+        for (int j : g.second)
+            if (jobs[j].max_depth != jobs[g.second[0]].max_depth)
+                fail(LJX_ERR_INVALID_ARGUMENT, "jobs of one scene must share max_depth");
+        const ljx_render_options &so = g.first->options;
+        if (so.integrator != LJX_INTEGRATOR_PATH && so.integrator != LJX_INTEGRATOR_VOLPATH)
+            fail(LJX_ERR_UNSUPPORTED, "batched jobs need the path or volpath integrator");
+        if (so.integrator == LJX_INTEGRATOR_VOLPATH && g.first->ds.envmap_light_id >= 0)
+            fail(LJX_ERR_UNSUPPORTED, "volpath assumes no environment map (handouts/homework2.tex:180)");
+    }
+    HIPX(hipSetDevice(dev));
+    // launch settings per scene
+    std::vector<ljx_exec_options> gex(groups.size());
+    for (size_t k = 0; k < groups.size(); k++) {
+        Scene &sc = *groups[k].first;
+        if (ex && ex->max_paths_in_flight > 0) gex[k] = *ex;
+        else {
+            if (!sc.tuned) autotune_impl(sc, nullptr);
+            gex[k] = sc.tuned_exec;
+        }
+        gex[k].collect_timing = 0;
+        gex[k].collect_traversal = 0;
+    }
+    // device job tables and accumulators
+    struct Buf {
+        void *p = nullptr;
+        ~Buf() {
+            if (p) hipFree(p);
+        }
+    };
+    std::vector<std::unique_ptr<Buf>> keep;
+    auto dmalloc = [&](size_t bytes) {
+        keep.push_back(std::make_unique<Buf>());
+        HIPX(hipMalloc(&keep.back()->p, std::max<size_t>(bytes, 16)));
+        return keep.back()->p;
+    };
+    struct GroupPlan {
+        std::vector<int> order;         // job indices, smallest work first
+        std::vector<BatchJob> table;
+        BatchJob *dtable = nullptr;
+        long long total = 0;
+        int capacity = 0, block = 256;
+    };
+    std::vector<GroupPlan> plans(groups.size());
+    uint64_t total_samples = 0;
+    for (size_t k = 0; k < groups.size(); k++) {
+        GroupPlan &gp = plans[k];
+        gp.order = groups[k].second;
+        auto work = [&](int j) {
+            const ljx_camera_desc &c = jobs[j].camera;
+            return (long long)((c.width + 7) / 8) * ((c.height + 7) / 8) * 64 * jobs[j].samples_per_pixel;
+        };
+        std::stable_sort(gp.order.begin(), gp.order.end(), [&](int a, int b) { return work(a) < work(b); });
+        for (int j : gp.order) {
+            const ljx_batch_job &J = jobs[j];
+            BatchJob b{};
+            b.camera = make_camera(J.camera);
+            b.W = J.camera.width;
+            b.H = J.camera.height;
+            b.ntx8 = (b.W + 7) / 8;
+            b.npix8 = b.ntx8 * ((b.H + 7) / 8) * 64;
+            b.spp = J.samples_per_pixel;
+            b.seed = J.seed;
+            b.work_base = gp.total;
+            b.accum = static_cast<double *>(dmalloc(size_t(b.W) * b.H * 3 * sizeof(double)));
+            b.out = J.rgb_out_device;
+            gp.total += work(j);
+            total_samples += uint64_t(b.W) * b.H * b.spp;
+            gp.table.push_back(b);
+        }
+        gp.dtable = static_cast<BatchJob *>(dmalloc(gp.table.size() * sizeof(BatchJob)));
+        HIPX(hipMemcpy(gp.dtable, gp.table.data(), gp.table.size() * sizeof(BatchJob), hipMemcpyHostToDevice));
+        long long want = gex[k].max_paths_in_flight > 0 ? gex[k].max_paths_in_flight : (1 << 22);
+        want = std::min<long long>(std::max<long long>(want, 64), 1 << kSlotBits);
+        gp.capacity = int(std::min<long long>(want, gp.total));
+        gp.block = gex[k].block_size > 0 ? gex[k].block_size : 256;
+        if (gp.block % 64 || gp.block > 256) fail(LJX_ERR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
+    }
+    hipStream_t cs = nullptr;
+    HIPX(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { hipStreamDestroy(s); }
+    } sg{cs};
+    JobEvents jev;
+    jev.ev.assign(njobs, nullptr);
+    for (auto &e : jev.ev) HIPX(hipEventCreate(&e));
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    HIPX(hipEventCreate(&ev_start));
+    HIPX(hipEventCreate(&ev_stop));
+    struct EvGuard {
+        hipEvent_t a, b;
+        ~EvGuard() {
+            hipEventDestroy(a);
+            hipEventDestroy(b);
+        }
+    } eg{ev_start, ev_stop};
+    // contexts first: allocation is not allowed during capture
+    for (size_t k = 0; k < groups.size(); k++) ensure_ctx(*groups[k].first, plans[k].capacity);
+    uint64_t passes = 0;
+    hipGraph_t graph = nullptr;
+    HIPX(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    try {
+        for (size_t k = 0; k < groups.size(); k++) {
+            Scene &sc = *groups[k].first;
+            GroupPlan &gp = plans[k];
+            const int J0 = gp.order[0];
+            const bool vol = sc.options.integrator == LJX_INTEGRATOR_VOLPATH;
+            PathBufs P = sc.rc->P;
+            P.spp = 0;
+            P.max_depth = jobs[J0].max_depth;
+            P.rr_depth = sc.options.rr_depth;
+            P.seed = 0;
+            P.volpath = vol ? 1 : 0;
+            P.max_null = sc.options.max_null_collisions;
+            P.rec_depth = 0;
+            P.jobs = gp.dtable;
+            P.njobs = int(gp.table.size());
+            P.sort_threshold = gex[k].sort_threshold < 0 ? INT_MAX : gex[k].sort_threshold;
+            P.W = P.H = P.ntx8 = 0;
+            const DeviceScene &S = sc.ds;
+            const int cap = gp.capacity;
+            const int sgrid = std::max(1, std::min(cap / gp.block + 1, sc.num_cu * 16 * 256 / gp.block));
+            const int tgrid = sc.rc->trace_grid;
+            for (int q = 0; q < P.njobs; q++) launch(k_job_clear, 256, 256, cs, (const BatchJob *)gp.dtable, q);
+            size_t next_done = 0;  // jobs complete in work order
+            for (long long w0 = 0; w0 < gp.total; w0 += cap) {
+                int n = int(std::min<long long>(cap, gp.total - w0));
+                P.work0 = w0;
+                P.pass_s0 = 0;
+                P.pass_ns = 1;
+                P.pass_q0 = 0;
+                P.pass_nq = n;
+                passes++;
+                launch(k_reset_counters, 1, 64, cs, P.ctr, 0, C_TOTAL);
+                launch(k_raygen, sgrid, 256, cs, S, P, 1);
+                if (vol) launch(k_vol_init, sgrid, 256, cs, S, P);
+                int cur = 0;
+                for (int b = 0; b < std::max(P.max_depth, 1); b++) {
+                    trace_stage(sc, P, cur, tgrid, cs, false, false);
+                    if (vol) vol_shade_all(S, P, cur, sgrid, cs, sc.has_spheres);
+                    else shade_all(sc, P, cur, sgrid, cs, false, gp.block);
+                    if (vol) vol_shadow(S, P, tgrid, cs, false, sc.has_spheres);
+                    else trace_stage(sc, P, cur, tgrid, cs, false, true);
+                    cur ^= 1;
+                }
+                launch(k_accumulate_batch, std::max(1, std::min(n / 256 + 1, sc.num_cu * 16)), 256, cs, P, n);
+                while (next_done < gp.table.size()) {
+                    const BatchJob &bj = gp.table[next_done];
+                    long long end = bj.work_base + (long long)bj.npix8 * bj.spp;
+                    if (end > w0 + n) break;
+                    launch(k_job_resolve, 256, 256, cs, (const BatchJob *)gp.dtable, int(next_done));
+                    HIPX(hipEventRecord(jev.ev[gp.order[next_done]], cs));
+                    next_done++;
+                }
+            }
+        }
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(cs, &g);
+        if (g) hipGraphDestroy(g);
+        throw;
+    }
+    HIPX(hipStreamEndCapture(cs, &graph));
+    struct GraphGuard {
+        hipGraph_t g;
+        hipGraphExec_t x = nullptr;
+        ~GraphGuard() {
+            if (x) hipGraphExecDestroy(x);
+            if (g) hipGraphDestroy(g);
+        }
+    } gg{graph};
+    size_t nodes = 0;
+    HIPX(hipGraphGetNodes(graph, nullptr, &nodes));
+    HIPX(hipGraphInstantiate(&gg.x, graph, nullptr, nullptr, 0));
+    auto t_built = std::chrono::steady_clock::now();
+    HIPX(hipEventRecord(ev_start, cs));
+    HIPX(hipGraphLaunch(gg.x, cs));
+    HIPX(hipEventRecord(ev_stop, cs));
+    HIPX(hipStreamSynchronize(cs));
+    float launch_ms = 0;
+    HIPX(hipEventElapsedTime(&launch_ms, ev_start, ev_stop));
+    if (per_job_ms)
+        for (size_t i = 0; i < njobs; i++) {
+            float ms = 0;
+            per_job_ms[i] = hipEventElapsedTime(&ms, ev_start, jev.ev[i]) == hipSuccess ? double(ms) : -1.0;
+        }
+    if (bst) {
+        std::memset(bst, 0, sizeof(*bst));
+        bst->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        bst->build_ms = std::chrono::duration<double, std::milli>(t_built - t0).count();
+        bst->launch_ms = launch_ms;
+        bst->samples = total_samples;
+        bst->passes = passes;
+        bst->graph_nodes = nodes;
+        bst->num_scenes = int(groups.size());
     }
 }
 
@@ -1096,6 +1389,20 @@ int ljx_render_guided_device(void *scene, void *guide_h, const ljx_render_option
         if (!guide_h) fail(LJX_ERR_INVALID_ARGUMENT, "guide handle is NULL");
         if (!rgb_out_device) fail(LJX_ERR_INVALID_ARGUMENT, "output is NULL");
         render_impl(*sc, opts, exec, rgb_out_device, stats, static_cast<guide::Net *>(guide_h), gopts);
+    });
+}
+
+int ljx_render_batch_device(const ljx_batch_job *jobs, size_t njobs, const ljx_exec_options *exec,
+                            double *per_job_ms, ljx_batch_stats *stats) {
+    return guarded([&]() { batch_impl(jobs, njobs, exec, per_job_ms, stats); });
+}
+
+int ljx_autotune(void *scene, const ljx_render_options *opts, ljx_exec_options *best) {
+    return guarded([&]() {
+        Scene *sc = as_scene(scene);
+        HIPX(hipSetDevice(sc->device));
+        ljx_exec_options b = autotune_impl(*sc, opts);
+        if (best) *best = b;
     });
 }
 

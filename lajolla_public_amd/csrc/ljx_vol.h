@@ -50,13 +50,7 @@ __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &
     D3 org = f3d(ro), dir = f3d(rd);
     int bounces = P.depth[pid];
     uint64_t st = P.rng[pid];
-    uint64_t stream;
-    {
-        int s, x, y;
-        bool valid;
-        slot_pixel(P, int(pid), s, x, y, valid);
-        stream = path_stream(P, s, x, y);
-    }
+    const uint64_t stream = slot_info(P, int(pid)).stream();
     uint64_t inc = pcg_inc(stream);
     float4 tp4 = P.tp[pid];
     D3 tp{tp4.x, tp4.y, tp4.z};
@@ -298,13 +292,7 @@ __global__ __launch_bounds__(256) void k_vol_shadow(DeviceScene S, PathBufs P) {
         int bounces = __float_as_int(sd.w);
         D3 p{so.x, so.y, so.z}, pp{sd.x, sd.y, sd.z};
         uint64_t nst = uint64_t(__float_as_uint(sx.z)) | (uint64_t(__float_as_uint(sx.w)) << 32);
-        uint64_t nstream;
-        {
-            int s, x, y;
-            bool valid;
-            slot_pixel(P, pid, s, x, y, valid);
-            nstream = path_stream(P, s, x, y) ^ kNeeStreamXor;
-        }
+        const uint64_t nstream = slot_info(P, pid).stream() ^ kNeeStreamXor;
         uint64_t ninc = pcg_inc(nstream);
         D3 dir_light = normalize(pp - p);
         D3 T{1, 1, 1}, p_nee{1, 1, 1}, p_dir{1, 1, 1};
@@ -395,7 +383,7 @@ __global__ void k_vol_init(DeviceScene S, PathBufs P) {
     int n = P.pass_ns * P.pass_nq;
     int stride = gridDim.x * blockDim.x;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        P.medium[i] = S.camera.medium_id;
+        P.medium[i] = P.jobs ? P.jobs[slot_info(P, i).job].camera.medium_id : S.camera.medium_id;
         float4 o = P.ray_o[i];
         P.nee_c[i] = make_float4(o.x, o.y, o.z, 0.f);
         P.mtp[i] = make_float4(1.f, 1.f, 1.f, 1.f);

@@ -48,6 +48,20 @@ def look_at(pos, look, up):
     return m
 
 
+def camera_desc(cam):
+    """ljx_camera_desc from a camera dict (cam_to_world, fov, width, height,
+    filter_type, filter_param, medium_id)."""
+    c = capi.CameraDesc()
+    c.cam_to_world = capi.D16(*np.asarray(cam["cam_to_world"], np.float64).reshape(16))
+    c.fov = cam["fov"]
+    c.width = cam["width"]
+    c.height = cam["height"]
+    c.filter_type = cam.get("filter_type", capi.FILTER_BOX)
+    c.filter_param = cam.get("filter_param", 1.0)
+    c.medium_id = cam.get("medium_id", -1)
+    return c
+
+
 # ------------------------------------------------------------------ the desc
 class SceneDesc:
     def __init__(self):
@@ -121,14 +135,7 @@ class SceneDesc:
     def to_c(self):
         keep = []
         d = capi.SceneDescC()
-        cam = self.camera
-        d.camera.cam_to_world = capi.D16(*np.asarray(cam["cam_to_world"], np.float64).reshape(16))
-        d.camera.fov = cam["fov"]
-        d.camera.width = cam["width"]
-        d.camera.height = cam["height"]
-        d.camera.filter_type = cam["filter_type"]
-        d.camera.filter_param = cam["filter_param"]
-        d.camera.medium_id = cam["medium_id"]
+        d.camera = camera_desc(self.camera)
 
         def ptr(a, ctype):
             if a is None:
