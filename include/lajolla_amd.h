@@ -298,6 +298,17 @@ int ljx_last_error(char *buf, size_t cap);
 /* Number of usable gfx950 devices.  Replaces rtcNewDevice (src/main.cpp:30). */
 int ljx_device_count(int *count);
 
+/* Reads a scene file (the Mitsuba-style XML subset of lajolla's parser, with
+ * OBJ meshes, constant textures and homogeneous media) into a descriptor
+ * owned by the library; free it with ljx_scene_desc_free.  Replaces the
+ * parsing half of parse_scene (src/parsers/parse_scene.cpp:1600-1617). */
+int ljx_scene_parse(const char *xml_path, ljx_scene_desc **desc_out);
+int ljx_scene_desc_free(ljx_scene_desc *desc);
+
+/* ljx_scene_parse followed by ljx_scene_create: the whole of parse_scene
+ * (src/parsers/parse_scene.h:9). */
+int ljx_scene_load(const char *xml_path, int device, void **scene_out);
+
 /* Builds the device scene: uploads geometry, builds the SAH BVH, light and
  * triangle sampling tables and the scene bounding sphere.  Replaces
  * Scene::Scene (src/scene.cpp:4-53) including rtcCommitScene. */

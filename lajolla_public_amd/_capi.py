@@ -11,6 +11,8 @@ LIB_PATH = os.path.join(_HERE, "liblajolla_amd.so")
 
 # status codes
 LJX_OK = 0
+(ERR_INVALID_ARGUMENT, ERR_OUT_OF_MEMORY, ERR_HIP, ERR_QUEUE_OVERFLOW, ERR_UNSUPPORTED,
+ ERR_NO_DEVICE) = range(1, 7)
 ERR_NAMES = {
     1: "invalid argument",
     2: "out of memory",
@@ -273,6 +275,9 @@ SIGNATURES = {
     "ljx_abi_version": ([], C.c_int),
     "ljx_last_error": ([C.c_char_p, C.c_size_t], C.c_int),
     "ljx_device_count": ([C.POINTER(C.c_int)], C.c_int),
+    "ljx_scene_parse": ([C.c_char_p, C.POINTER(C.POINTER(SceneDescC))], C.c_int),
+    "ljx_scene_desc_free": ([C.POINTER(SceneDescC)], C.c_int),
+    "ljx_scene_load": ([C.c_char_p, C.c_int, C.POINTER(VP)], C.c_int),
     "ljx_scene_create": ([C.POINTER(SceneDescC), C.c_int, C.POINTER(VP)], C.c_int),
     "ljx_scene_destroy": ([VP], C.c_int),
     "ljx_scene_bounds": ([VP, C.POINTER(C.c_double)], C.c_int),

@@ -16,7 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["ljx_lib.hip", "ljx_guide_net.hip"]
-CXX_SOURCES = ["ljx_bvh.cpp"]
+CXX_SOURCES = ["ljx_bvh.cpp", "ljx_parse.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
           "-Wno-unused-result"]
@@ -46,6 +46,11 @@ def build(verbose=True):
     tmp = OUT + ".tmp"
     run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, OUT)
+    # the command-line renderer, a host program over the C++ API
+    cli = os.path.join(HERE, "lajolla_amd")
+    run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(HERE, "..", "include"), os.path.join(HERE, "host", "main.cpp"),
+         "-L" + HERE, "-llajolla_amd", "-Wl,-rpath,$ORIGIN", "-o", cli + ".tmp"])
+    os.replace(cli + ".tmp", cli)
     return OUT
 
 

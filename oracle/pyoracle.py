@@ -87,6 +87,18 @@ class OracleScene:
         self.width = desc.camera["width"]
         self.height = desc.camera["height"]
 
+    @classmethod
+    def from_c(cls, desc_c):
+        """From an ljx_scene_desc structure (e.g. one ljx_scene_parse made)."""
+        self = cls.__new__(cls)
+        self.desc = None
+        h = C.c_void_p()
+        _check(lib().orc_scene_create(C.byref(desc_c), C.byref(h)))
+        self.h = h
+        self.width = desc_c.camera.width
+        self.height = desc_c.camera.height
+        return self
+
     def close(self):
         if self.h:
             lib().orc_scene_destroy(self.h)

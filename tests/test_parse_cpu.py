@@ -1,0 +1,208 @@
+"""The scene-file reader (ljx_scene_parse): XML subset + OBJ, no GPU needed.
+
+Checks against what the file says and what the conversions fix:
+  * <default> substitution, integrator and sampler settings;
+  * fovAxis "smaller" on a landscape film converts a y field of view to x
+    (2 atan(tan(fov/2) aspect), src/parsers/parse_scene.cpp:830-843);
+  * Beckmann alpha -> roughness sqrt(alpha), IOR ratio int/ext;
+  * sRGB hex -> linear (IEC 61966-2-1 curve);
+  * a flat 0.5 spectrum -> 0.5 x the XYZ(1,1,1) white in linear sRGB
+    primaries, within the accuracy of the analytic CIE fits;
+  * OBJ: quads split into two triangles, one vertex per distinct (v, vt, vn),
+    vt flipped to 1 - t, normals transformed by the inverse transpose;
+    missing normals computed (a flat floor gets +y everywhere);
+  * transforms: scale, rotate, translate composed in document order;
+  * unsupported inputs fail with LJX_ERR_UNSUPPORTED, malformed ones with
+    LJX_ERR_INVALID_ARGUMENT, without crashing.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+import pytest
+
+from lajolla_public_amd import _capi as capi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOM = os.path.join(HERE, "golden", "scenes", "room.xml")
+
+
+def parse(path):
+    lib = capi.load()
+    d = C.POINTER(capi.SceneDescC)()
+    rc = lib.ljx_scene_parse(path.encode(), C.byref(d))
+    return rc, d, lib
+
+
+@pytest.fixture(scope="module")
+def room():
+    rc, d, lib = parse(ROOM)
+    assert rc == 0
+    yield d.contents
+    lib.ljx_scene_desc_free(d)
+
+
+def mesh_arrays(s):
+    nv, nt = s.num_vertices, s.num_triangles
+    P = np.ctypeslib.as_array(s.positions, (nv * 3,)).reshape(nv, 3).copy()
+    I = np.ctypeslib.as_array(s.indices, (nt * 3,)).reshape(nt, 3).copy()
+    N = np.ctypeslib.as_array(s.normals, (nv * 3,)).reshape(nv, 3).copy() if s.normals else None
+    UV = np.ctypeslib.as_array(s.uvs, (nv * 2,)).reshape(nv, 2).copy() if s.uvs else None
+    return P, I, N, UV
+
+
+def test_options_and_camera(room):
+    o = room.options
+    assert o.integrator == capi.INTEGRATOR_PATH
+    assert (o.samples_per_pixel, o.max_depth, o.rr_depth) == (16, 5, 3)
+    c = room.camera
+    assert (c.width, c.height) == (48, 32)
+    assert c.filter_type == capi.FILTER_GAUSSIAN and abs(c.filter_param - 0.6) < 1e-7
+    expect = math.degrees(2 * math.atan(math.tan(math.radians(40) / 2) * 48 / 32))
+    assert abs(c.fov - expect) < 1e-9
+    m = np.array(c.cam_to_world[:]).reshape(4, 4)
+    np.testing.assert_allclose(m[:3, 3], [0, 1, 4.5])
+    np.testing.assert_allclose(m[:3, 2], [0, 0, -1], atol=1e-12)  # looking down -z
+
+
+def test_materials(room):
+    mats = [room.materials[i] for i in range(room.num_materials)]
+    assert [m.type for m in mats] == [capi.MAT_LAMBERTIAN] * 3 + [capi.MAT_ROUGH_PLASTIC, capi.MAT_DISNEY_BSDF]
+    np.testing.assert_allclose(mats[0].color_a[:], [0.7, 0.7, 0.7], rtol=1e-7)
+    lin = lambda c: c / 12.92 if c <= 0.04045 else ((c + 0.055) / 1.055) ** 2.4
+    expect = [lin(np.float32(v / 255.0)) for v in (0xb0, 0x1a, 0x1a)]
+    np.testing.assert_allclose(mats[1].color_a[:], expect, rtol=1e-6)
+    white = np.array([3.240479 - 1.537150 - 0.498535, -0.969256 + 1.875991 + 0.041556,
+                      0.055648 - 0.204043 + 1.057311])
+    np.testing.assert_allclose(mats[2].color_a[:], 0.5 * white, rtol=0.03)
+    p = mats[3]
+    assert abs(p.roughness - 0.3) < 1e-7 and abs(p.eta - 1.6) < 1e-7
+    np.testing.assert_allclose(p.color_a[:], [0.2, 0.4, 0.6], rtol=1e-7)
+    np.testing.assert_allclose(p.color_b[:], [1, 1, 1])
+    d = mats[4]
+    assert (round(d.metallic, 6), round(d.roughness, 6), round(d.clearcoat, 6), round(d.sheen, 6)) == \
+           (0.3, 0.4, 0.5, 0.25)
+    assert d.specular == 0.5 and d.sheen_tint == 0.5 and d.clearcoat_gloss == 1.0 and d.eta == 1.5
+
+
+def test_obj_meshes(room):
+    assert room.num_shapes == 5
+    floor = room.shapes[0]
+    P, I, N, UV = mesh_arrays(floor)
+    assert floor.num_triangles == 4 and floor.num_vertices == 6 and UV is None
+    np.testing.assert_allclose(N, np.tile([0, 1, 0], (6, 1)), atol=1e-7)  # computed normals
+    box = room.shapes[1]
+    P, I, N, UV = mesh_arrays(box)
+    # 6 quads -> 12 triangles; 24 distinct (v, vt, vn) corners
+    assert box.num_triangles == 12 and box.num_vertices == 24
+    assert UV is not None and set(np.unique(UV)) <= {0.0, 1.0}
+    # scale 0.4, rotate 30 deg about y, translate (-0.7, 0.4, 0.2)
+    a = math.radians(30)
+    R = np.array([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])
+    corner = R @ (0.4 * np.array([1.0, 1.0, 1.0])) + [-0.7, 0.4, 0.2]
+    assert np.min(np.linalg.norm(P - corner, axis=1)) < 1e-6
+    # transformed normals stay unit and perpendicular to their faces
+    np.testing.assert_allclose(np.linalg.norm(N, axis=1), 1, atol=1e-6)
+    for t in I:
+        fn = np.cross(P[t[1]] - P[t[0]], P[t[2]] - P[t[0]])
+        fn /= np.linalg.norm(fn)
+        assert abs(abs(fn @ N[t[0]]) - 1) < 1e-5
+
+
+def test_sphere_rectangle_and_emitter(room):
+    sph = room.shapes[2]
+    assert sph.type == capi.SHAPE_SPHERE
+    np.testing.assert_allclose([sph.center[0], sph.center[1], sph.center[2], sph.radius], [0.7, 0.45, 0.3, 0.45],
+                               rtol=1e-7)
+    assert sph.material_id == 4
+    light = room.shapes[4]
+    assert light.area_light_id == 0 and room.num_lights == 1
+    l = room.lights[0]
+    assert l.type == capi.LIGHT_DIFFUSE_AREA and l.shape_id == 4
+    np.testing.assert_allclose(l.intensity[:], [12, 10, 8])
+    P, I, N, _ = mesh_arrays(light)
+    # rotated 90 deg about x: the rectangle's +z normal points to -y (down)
+    np.testing.assert_allclose(N, np.tile([0, -1, 0], (4, 1)), atol=1e-7)
+    np.testing.assert_allclose(P[:, 1], 2.2, atol=1e-6)
+
+
+def write(tmp_path, body, name="s.xml"):
+    p = tmp_path / name
+    p.write_text(f'<?xml version="1.0"?>\n<scene version="0.6.0">{body}</scene>\n')
+    return str(p)
+
+
+def test_media_points_directional_and_volpath(tmp_path):
+    body = """
+    <integrator type="volpath"><integer name="maxDepth" value="7"/><integer name="version" value="6"/>
+      <integer name="maxNullCollisions" value="99"/></integrator>
+    <medium type="homogeneous" id="fog">
+      <rgb name="sigmaA" value="0.1, 0.2, 0.3"/><rgb name="sigmaS" value="1, 2, 3"/>
+      <float name="scale" value="2"/><phase type="hg"><float name="g" value="0.7"/></phase>
+    </medium>
+    <sensor type="perspective"><ref id="fog"/><float name="fov" value="30"/></sensor>
+    <shape type="sphere"><ref name="interior" id="fog"/><float name="radius" value="2"/></shape>
+    <emitter type="point"><point name="position" x="1" y="2" z="3"/><rgb name="intensity" value="4, 4, 4"/></emitter>
+    <emitter type="directional"><vector name="direction" x="0" y="-1" z="0"/>
+      <rgb name="irradiance" value="1, 1, 1"/></emitter>
+    """
+    rc, d, lib = parse(write(tmp_path, body))
+    assert rc == 0
+    s = d.contents
+    assert s.options.integrator == capi.INTEGRATOR_VOLPATH
+    assert (s.options.max_depth, s.options.vol_path_version, s.options.max_null_collisions) == (7, 6, 99)
+    assert s.num_media == 1 and s.camera.medium_id == 0
+    m = s.media[0]
+    np.testing.assert_allclose(m.sigma_a[:], [0.2, 0.4, 0.6], rtol=1e-6)
+    np.testing.assert_allclose(m.sigma_s[:], [2, 4, 6], rtol=1e-6)
+    assert m.phase_type == capi.PHASE_HENYEY_GREENSTEIN and abs(m.g - 0.7) < 1e-7
+    assert s.shapes[0].interior_medium_id == 0 and s.shapes[0].material_id == -1
+    # point light: a sphere of radius 1e-4 with intensity * 4 pi / area
+    pt = s.shapes[1]
+    assert pt.type == capi.SHAPE_SPHERE and pt.radius == 1e-4
+    np.testing.assert_allclose(s.lights[0].intensity[:], [4 / 1e-8] * 3, rtol=1e-9)
+    # directional: a 1e-3 quad 1e3 away against the direction
+    q = s.shapes[2]
+    P, I, N, _ = mesh_arrays(q)
+    np.testing.assert_allclose(P.mean(0), [0, 1000, 0], atol=1e-3)
+    np.testing.assert_allclose(N, np.tile([0, -1, 0], (4, 1)), atol=1e-7)
+    lib.ljx_scene_desc_free(d)
+
+
+@pytest.mark.parametrize("body,code", [
+    ('<texture type="bitmap" id="t"/>', capi.ERR_UNSUPPORTED),
+    ('<shape type="ply"><string name="filename" value="x.ply"/></shape>', capi.ERR_UNSUPPORTED),
+    ('<emitter type="envmap"><string name="filename" value="sky.exr"/></emitter>', capi.ERR_UNSUPPORTED),
+    ('<medium type="heterogeneous" id="m"/>', capi.ERR_UNSUPPORTED),
+    ('<bsdf type="nonsense" id="b"/>', capi.ERR_INVALID_ARGUMENT),
+    ('<shape type="obj"><ref id="missing"/><string name="filename" value="a.obj"/></shape>',
+     capi.ERR_INVALID_ARGUMENT),
+    ('<shape type="obj"><string name="filename" value="does_not_exist.obj"/></shape>', capi.ERR_INVALID_ARGUMENT),
+    ('<integrator type="path"><integer name="maxDepth" value="x"/></integrator>', capi.ERR_INVALID_ARGUMENT),
+    ('<sensor type="perspective"><float name="fov" value="30"', capi.ERR_INVALID_ARGUMENT),
+])
+def test_errors(tmp_path, body, code):
+    rc, d, lib = parse(write(tmp_path, body))
+    assert rc == code
+    buf = C.create_string_buffer(512)
+    lib.ljx_last_error(buf, 512)
+    assert buf.value
+
+
+REF_CBOX = "/root/reference/scenes/cbox/cbox.xml"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_CBOX), reason="reference scenes not present")
+def test_reference_cornell_box_parses():
+    rc, d, lib = parse(REF_CBOX)
+    assert rc == 0
+    s = d.contents
+    assert s.camera.width == 512 and s.options.samples_per_pixel == 4 and s.options.max_depth == -1
+    assert s.num_lights == 1 and s.num_shapes >= 8
+    tris = sum(s.shapes[i].num_triangles for i in range(s.num_shapes))
+    assert tris > 20
+    # the light's spectrum (400:0, 500:8, 600:15.6, 700:18.4) is warm: R > G > B
+    r, g, b = s.lights[0].intensity[:]
+    assert r > g > b > 0
+    lib.ljx_scene_desc_free(d)
