@@ -1010,13 +1010,11 @@ ljx_exec_options autotune_impl(Scene &sc, const ljx_render_options *o) {
 }
 
 // ------------------------------------------------------------ batch render
-struct JobEvents {
-    std::vector<hipEvent_t> ev;
-    ~JobEvents() {
-        for (auto e : ev)
-            if (e) hipEventDestroy(e);
-    }
-};
+// Device wall-clock stamp (the constant-rate real-time counter), written by
+// one lane; stamps inside a captured graph time the jobs of a batch.
+__global__ void k_stamp(unsigned long long *dst) {
+    if (threadIdx.x == 0) dst[0] = wall_clock64();
+}
 
 void batch_impl(const ljx_batch_job *jobs, size_t njobs, const ljx_exec_options *ex, double *per_job_ms,
                 ljx_batch_stats *bst) {
@@ -1127,9 +1125,10 @@ void batch_impl(const ljx_batch_job *jobs, size_t njobs, const ljx_exec_options 
         hipStream_t s;
         ~StreamGuard() { hipStreamDestroy(s); }
     } sg{cs};
-    JobEvents jev;
-    jev.ev.assign(njobs, nullptr);
-    for (auto &e : jev.ev) HIPX(hipEventCreate(&e));
+    // stamps: [0] graph start, [1 + i] completion of job i
+    auto *stamps = static_cast<unsigned long long *>(dmalloc((njobs + 1) * sizeof(unsigned long long)));
+    int clock_khz = 0;
+    HIPX(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, dev));
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     HIPX(hipEventCreate(&ev_start));
     HIPX(hipEventCreate(&ev_stop));
@@ -1146,6 +1145,7 @@ void batch_impl(const ljx_batch_job *jobs, size_t njobs, const ljx_exec_options 
     hipGraph_t graph = nullptr;
     HIPX(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     try {
+        launch(k_stamp, 1, 64, cs, stamps);
         for (size_t k = 0; k < groups.size(); k++) {
             Scene &sc = *groups[k].first;
             GroupPlan &gp = plans[k];
@@ -1195,7 +1195,7 @@ void batch_impl(const ljx_batch_job *jobs, size_t njobs, const ljx_exec_options 
                     long long end = bj.work_base + (long long)bj.npix8 * bj.spp;
                     if (end > w0 + n) break;
                     launch(k_job_resolve, 256, 256, cs, (const BatchJob *)gp.dtable, int(next_done));
-                    HIPX(hipEventRecord(jev.ev[gp.order[next_done]], cs));
+                    launch(k_stamp, 1, 64, cs, stamps + 1 + gp.order[next_done]);
                     next_done++;
                 }
             }
@@ -1225,11 +1225,12 @@ void batch_impl(const ljx_batch_job *jobs, size_t njobs, const ljx_exec_options 
     HIPX(hipStreamSynchronize(cs));
     float launch_ms = 0;
     HIPX(hipEventElapsedTime(&launch_ms, ev_start, ev_stop));
-    if (per_job_ms)
-        for (size_t i = 0; i < njobs; i++) {
-            float ms = 0;
-            per_job_ms[i] = hipEventElapsedTime(&ms, ev_start, jev.ev[i]) == hipSuccess ? double(ms) : -1.0;
-        }
+    if (per_job_ms) {
+        std::vector<unsigned long long> st(njobs + 1);
+        HIPX(hipMemcpy(st.data(), stamps, st.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < njobs; i++)
+            per_job_ms[i] = clock_khz > 0 ? double(st[1 + i] - st[0]) / double(clock_khz) : -1.0;
+    }
     if (bst) {
         std::memset(bst, 0, sizeof(*bst));
         bst->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -519,6 +519,44 @@ def terrain_gap(seed=7, width=1920, height=1080, spp=64, max_depth=8, gap_frac=0
     return s
 
 
+def orbit_camera(desc, rng, width, height, yaw_deg=20.0, pitch_deg=5.0, dist_range=(0.85, 1.15)):
+    """A seeded camera pose around the scene camera's view: yaw and pitch
+    about the point 10 units ahead, distance scaled; the fov is kept."""
+    cam = dict(desc.camera)
+    m = np.asarray(cam["cam_to_world"], np.float64)
+    pos, fwd = m[:3, 3], m[:3, 2]
+    target = pos + 10.0 * fwd
+    off = pos - target
+    yaw = math.radians(rng.uniform(-yaw_deg, yaw_deg))
+    pitch = math.radians(rng.uniform(-pitch_deg, pitch_deg))
+    cy, sy = math.cos(yaw), math.sin(yaw)
+    off = np.array([cy * off[0] + sy * off[2], off[1], -sy * off[0] + cy * off[2]])
+    r = np.linalg.norm(off)
+    el = math.asin(np.clip(off[1] / r, -1, 1)) + pitch
+    horiz = math.hypot(off[0], off[2])
+    off = np.array([off[0] / horiz * r * math.cos(el), r * math.sin(el), off[2] / horiz * r * math.cos(el)])
+    off *= rng.uniform(*dist_range)
+    cam["cam_to_world"] = look_at(target + off, target, (0, 1, 0))
+    cam["width"], cam["height"] = int(width), int(height)
+    return cam
+
+
+def batch_jobs(scenes_and_descs, n_jobs=96, seed=3, res_range=(64, 1024), spps=(4, 16, 64, 256), max_depth=8):
+    """The batched-rendering workload: n_jobs jobs, each on one of the given
+    (scene, desc) pairs in turn, with a seeded orbit pose, width and height
+    log-uniform in res_range, spp drawn from spps."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    lo, hi = math.log(res_range[0]), math.log(res_range[1])
+    for k in range(n_jobs):
+        sc, desc = scenes_and_descs[k % len(scenes_and_descs)]
+        W = int(round(math.exp(rng.uniform(lo, hi))))
+        H = int(round(math.exp(rng.uniform(lo, hi))))
+        jobs.append(dict(scene=sc, camera=orbit_camera(desc, rng, W, H), spp=int(rng.choice(spps)),
+                         max_depth=max_depth, seed=int(rng.integers(1, 1 << 62))))
+    return jobs
+
+
 def nested_media(seed=11, g=0.7, width=1024, height=1024, spp=128, max_depth=16, sphere_res=224):
     """Volumetric scene: a sphere_res^2*2-triangle index-matched sphere holding
     a homogeneous medium (sigma_a ~ U[0.01,0.5]^3, sigma_s ~ U[0.5,4]^3),
