@@ -223,25 +223,34 @@ def batched_jobs(sc_c1, desc_c1, n_jobs=96):
     }
 
 
-def rel_mse(img, ref):
+def rel_mse(img, ref, trim=0.0):
+    """mean((x - ref)^2 / (ref^2 + 0.01)) over pixels and channels; with trim,
+    the largest per-pixel errors (that fraction of the pixels) are left out."""
     import numpy as np
-    return float(np.mean((img.astype(np.float64) - ref) ** 2 / (ref.astype(np.float64) ** 2 + 1e-2)))
+    e = ((img.astype(np.float64) - ref) ** 2 / (ref.astype(np.float64) ** 2 + 1e-2)).mean(axis=2).ravel()
+    if trim > 0:
+        e = np.sort(e)[: int(round(len(e) * (1 - trim)))]
+    return float(e.mean())
 
 
-def guided_vs_plain(ref_spp, plain_spp=64, train_spp=8, steps_per_pass=6):
+TRIM = 1e-4  # the 0.01% largest per-pixel errors (207 pixels at 1080p)
+
+
+def guided_vs_plain(ref_spp, plain_spp=64, train_spp=8, steps_per_pass=6, plain_seeds=3):
     """relMSE at equal wall time, learned guiding vs the plain tracer, on the
     guiding workload (terrain_gap: the large-mesh scene under a ceiling with a
     5% gap, 90% of the light power behind it), 1920x1080, depth 8.  Each
     guided render starts from an untrained network: its training passes and
     Adam steps are inside the timed wall time, and their samples stay in the
     image.  relMSE = mean((x - ref)^2 / (ref^2 + 0.01)) against a plain
-    render at ref_spp."""
+    render at ref_spp; the plain figure is the mean over plain_seeds renders.
+    Both are also given with the 0.01% largest per-pixel errors left out,
+    which shows how much of the standard figure is rare outliers."""
     import numpy as np
     from lajolla_public_amd import api, scenes
     t0 = time.time()
     desc = scenes.terrain_gap(seed=7)
     sc = api.Scene(desc)
-    W, H = desc.camera["width"], desc.camera["height"]
     log = lambda m: print(f"[bench] guiding: {m}", file=sys.stderr, flush=True)
     log(f"scene ready in {time.time() - t0:.1f}s ({desc.num_triangles} triangles)")
 
@@ -253,11 +262,6 @@ def guided_vs_plain(ref_spp, plain_spp=64, train_spp=8, steps_per_pass=6):
     log(f"reference {ref_spp} spp in {time.time() - t0:.1f}s")
     sc.render(options=opts(4, 1))  # warm-up
 
-    def plain(spp, seed):
-        t = time.perf_counter()
-        img = sc.render(options=opts(spp, seed))
-        return (time.perf_counter() - t) * 1000, rel_mse(img, ref)
-
     def guided(spp, seed):
         g = api.Guide(seed=seed)
         go = api.guide_options(mix_prob=0.5, train_spp=min(train_spp, spp), steps_per_pass=steps_per_pass)
@@ -265,44 +269,61 @@ def guided_vs_plain(ref_spp, plain_spp=64, train_spp=8, steps_per_pass=6):
         img, st = api.render_guided(sc, g, opts(spp, seed), gopts=go, stats=True)
         ms = (time.perf_counter() - t) * 1000
         g.close()
-        return ms, rel_mse(img, ref), st
+        return ms, img, st
 
     guided(16, 7)  # warm-up (first launch of the guided kernels)
-    p_ms, p_err = plain(plain_spp, 1)
-    log(f"plain {plain_spp} spp: {p_ms:.0f} ms, relMSE {p_err:.5f}")
+    plain = []
+    for k in range(plain_seeds):
+        t = time.perf_counter()
+        img = sc.render(options=opts(plain_spp, 1 + k))
+        ms = (time.perf_counter() - t) * 1000
+        plain.append({"seed": 1 + k, "ms": round(ms, 1), "relmse": round(rel_mse(img, ref), 6),
+                      "relmse_trimmed": round(rel_mse(img, ref, TRIM), 6)})
+        log(f"plain {plain_spp} spp seed {1 + k}: {ms:.0f} ms, relMSE {plain[-1]['relmse']:.5f} "
+            f"(trimmed {plain[-1]['relmse_trimmed']:.5f})")
+    p_ms = float(np.mean([p["ms"] for p in plain]))
+    p_err = float(np.mean([p["relmse"] for p in plain]))
+    p_trim = float(np.mean([p["relmse_trimmed"] for p in plain]))
     curve = []
     for spp in (8, 16, 24, 32, 48, 64):
-        ms, err, st = guided(spp, 100 + spp)
-        curve.append({"spp": spp, "ms": round(ms, 1), "relmse": round(err, 6), "train_samples": st["train_samples"],
-                      "train_steps": st["train_steps"], "train_ms": round(st["train_ms"], 1)})
-        log(f"guided {spp} spp: {ms:.0f} ms, relMSE {err:.5f}, {st['train_steps']} Adam steps on "
-            f"{st['train_samples']} samples")
-    # equal wall time: interpolate the guided relMSE at the plain render's time (log-log)
+        ms, img, st = guided(spp, 100 + spp)
+        curve.append({"spp": spp, "ms": round(ms, 1), "relmse": round(rel_mse(img, ref), 6),
+                      "relmse_trimmed": round(rel_mse(img, ref, TRIM), 6), "train_samples": st["train_samples"],
+                      "train_steps": st["train_steps"]})
+        log(f"guided {spp} spp: {ms:.0f} ms, relMSE {curve[-1]['relmse']:.5f} (trimmed "
+            f"{curve[-1]['relmse_trimmed']:.5f}), {st['train_steps']} Adam steps on {st['train_samples']} samples")
     t = np.array([c["ms"] for c in curve])
-    e = np.array([c["relmse"] for c in curve])
-    order = np.argsort(t)
-    t, e = t[order], e[order]
-    err_eq = float(np.exp(np.interp(np.log(p_ms), np.log(t), np.log(e))))
-    # wall time at which the guided render reaches the plain render's relMSE
-    if e.min() <= p_err:
-        k = int(np.argmax(e <= p_err))
+
+    def at_time(key):
+        e = np.array([c[key] for c in curve])
+        return float(np.exp(np.interp(np.log(p_ms), np.log(t), np.log(e))))
+
+    def time_to(key, target):
+        e = np.array([c[key] for c in curve])
+        if e.min() > target:
+            return None
+        k = int(np.argmax(e <= target))
         if k == 0:
-            t_match = float(t[0])
-        else:
-            f = (np.log(p_err) - np.log(e[k - 1])) / (np.log(e[k]) - np.log(e[k - 1]))
-            t_match = float(np.exp(np.log(t[k - 1]) + f * (np.log(t[k]) - np.log(t[k - 1]))))
-    else:
-        t_match = None
+            return float(t[0])
+        f = (np.log(target) - np.log(e[k - 1])) / (np.log(e[k]) - np.log(e[k - 1]))
+        return float(np.exp(np.log(t[k - 1]) + f * (np.log(t[k]) - np.log(t[k - 1]))))
+
+    eq, eq_trim = at_time("relmse"), at_time("relmse_trimmed")
+    t_match, t_match_trim = time_to("relmse", p_err), time_to("relmse_trimmed", p_trim)
     return {
         "workload": "terrain_gap: terrain_spheres (2,308,746 triangles) under a ceiling with a 5% gap; a 1x1 "
                     "emitter above the gap carries 90% of the light power; 1920x1080, depth 8",
         "reference_spp": ref_spp,
-        "plain": {"spp": plain_spp, "ms": round(p_ms, 1), "relmse": round(p_err, 6)},
+        "plain": {"spp": plain_spp, "ms": round(p_ms, 1), "relmse": round(p_err, 6),
+                  "relmse_trimmed": round(p_trim, 6), "seeds": plain},
         "guided_curve": curve,
-        "guided_relmse_at_equal_time": round(err_eq, 6),
-        "relmse_ratio_plain_over_guided_equal_time": round(p_err / err_eq, 3),
+        "guided_relmse_at_equal_time": round(eq, 6),
+        "relmse_ratio_plain_over_guided_equal_time": round(p_err / eq, 3),
         "guided_time_to_plain_relmse_ms": round(t_match, 1) if t_match else None,
         "time_ratio_guided_over_plain": round(t_match / p_ms, 3) if t_match else None,
+        "trimmed": {"trim_fraction": TRIM, "guided_relmse_at_equal_time": round(eq_trim, 6),
+                    "relmse_ratio_plain_over_guided_equal_time": round(p_trim / eq_trim, 3),
+                    "time_ratio_guided_over_plain": round(t_match_trim / p_ms, 3) if t_match_trim else None},
         "mix_prob": 0.5,
         "train": {"train_spp": train_spp, "steps_per_pass": steps_per_pass, "batch": 1 << 20, "optimizer": "adam",
                   "lr": 1e-3},
