@@ -215,6 +215,7 @@ class ExecOptions(C.Structure):
         ("max_paths_in_flight", C.c_int32),
         ("block_size", C.c_int32),
         ("sort_threshold", C.c_int32),
+        ("trace_blocks_per_cu", C.c_int32),
     ]
 
 

@@ -85,8 +85,10 @@ def stats_dict(st):
     return {name: getattr(st, name) for name, _ in capi.RenderStats._fields_}
 
 
-def exec_options(collect_timing=0, collect_traversal=0, max_paths_in_flight=0, block_size=0, sort_threshold=0):
+def exec_options(collect_timing=0, collect_traversal=0, max_paths_in_flight=0, block_size=0, sort_threshold=0,
+                 trace_blocks_per_cu=0):
     e = capi.ExecOptions()
+    e.trace_blocks_per_cu = trace_blocks_per_cu
     e.collect_timing = collect_timing
     e.collect_traversal = collect_traversal
     e.max_paths_in_flight = max_paths_in_flight

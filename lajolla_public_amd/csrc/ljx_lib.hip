@@ -598,7 +598,12 @@ Scene *as_scene(void *p) {
     return static_cast<Scene *>(p);
 }
 
-int trace_grid_for(const Scene &sc) { return std::max(1, sc.num_cu * 4); }
+// Persistent traversal grid: blocks_per_cu blocks of 256 lanes per CU (the
+// LDS stack allows eight).
+int trace_grid_for(const Scene &sc, int blocks_per_cu = 0) {
+    int b = blocks_per_cu > 0 ? std::min(blocks_per_cu, 16) : 8;
+    return std::max(1, sc.num_cu * b);
+}
 
 struct SpillBuf {
     int *p = nullptr;
@@ -638,12 +643,14 @@ struct Timer {
     }
 };
 
-void ensure_ctx(Scene &sc, int capacity) {
-    if (sc.rc && sc.rc->capacity >= capacity && sc.rc_alloc_W == sc.width && sc.rc_alloc_H == sc.height) return;
+void ensure_ctx(Scene &sc, int capacity, int trace_grid) {
+    if (sc.rc && sc.rc->capacity >= capacity && sc.rc->trace_grid == trace_grid && sc.rc_alloc_W == sc.width &&
+        sc.rc_alloc_H == sc.height)
+        return;
     sc.release_ctx();
     auto rc = std::make_unique<RenderCtx>();
     rc->capacity = capacity;
-    rc->trace_grid = trace_grid_for(sc);
+    rc->trace_grid = trace_grid;
     PathBufs &P = rc->P;
     size_t C = size_t(capacity);
     P.capacity = capacity;
@@ -804,7 +811,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
         nq_per_pass = int((want / 64) * 64);
     }
     capacity = ns_per_pass * nq_per_pass;
-    ensure_ctx(sc, capacity);
+    ensure_ctx(sc, std::max(capacity, sc.rc ? sc.rc->capacity : 0), trace_grid_for(sc, e.trace_blocks_per_cu));
     RenderCtx &rc = *sc.rc;
     PathBufs P = rc.P;
     P.W = W;
@@ -1004,6 +1011,12 @@ ljx_exec_options autotune_impl(Scene &sc, const ljx_render_options *o) {
         double ms = trial(e);
         if (ms < best_ms) best_ms = ms, best = e;
     }
+    for (int bpc : {4, 6}) {
+        ljx_exec_options e = best;
+        e.trace_blocks_per_cu = bpc;
+        double ms = trial(e);
+        if (ms < best_ms) best_ms = ms, best = e;
+    }
     sc.tuned = true;
     sc.tuned_exec = best;
     return best;
@@ -1140,7 +1153,8 @@ void batch_impl(const ljx_batch_job *jobs, size_t njobs, const ljx_exec_options 
         }
     } eg{ev_start, ev_stop};
     // contexts first: allocation is not allowed during capture
-    for (size_t k = 0; k < groups.size(); k++) ensure_ctx(*groups[k].first, plans[k].capacity);
+    for (size_t k = 0; k < groups.size(); k++)
+        ensure_ctx(*groups[k].first, plans[k].capacity, trace_grid_for(*groups[k].first, gex[k].trace_blocks_per_cu));
     uint64_t passes = 0;
     hipGraph_t graph = nullptr;
     HIPX(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));

@@ -14,8 +14,10 @@
 
 namespace ljx {
 
-constexpr int kStackLds = 24;     // per-lane LDS stack entries
-constexpr int kStackSpill = 96;   // per-lane spill entries (tree depth <= kStackLds + kStackSpill)
+// 20 LDS entries x 256 lanes = 20 KB per block: eight traversal blocks fit
+// the 160 KB of LDS of a CU, i.e. eight waves per SIMD at 64 VGPRs.
+constexpr int kStackLds = 20;     // per-lane LDS stack entries
+constexpr int kStackSpill = 100;  // per-lane spill entries (tree depth <= kStackLds + kStackSpill)
 constexpr int kMaxTreeDepth = kStackLds + kStackSpill - 2;
 constexpr int kPrimTri = 0, kPrimSphere = 1;
 
