@@ -109,13 +109,15 @@ def main():
     grays = (ext_rays + sh_rays) / elapsed / 1e9
 
     # roofline of the dominant kernel (closest-hit traversal): algorithmic
-    # bytes = 48 B per ray (32 B ray in, 16 B hit out) + 64 B per visited node
-    # (two 32 B child boxes) + 36 B per primitive test (three FP32 vertices)
+    # bytes = 48 B per ray (32 B ray in, 16 B hit out) + 112 B per visited
+    # 4-wide node (four child boxes and their references) + 36 B per
+    # primitive test (three FP32 vertices)
     ext_ms = sum(s["extend_ms"] for s in stats)
     ext_launches = sum(s["extend_launches"] for s in stats)
     nodes_per_ray = inst["node_visits"] / max(1, inst["extension_rays"])
     tris_per_ray = inst["tri_tests"] / max(1, inst["extension_rays"])
-    bytes_per_ray = 48 + 64 * nodes_per_ray + 36 * tris_per_ray
+    node_bytes = inst["node_bytes"]
+    bytes_per_ray = 48 + node_bytes * nodes_per_ray + 36 * tris_per_ray
     alg_bytes = ext_rays * bytes_per_ray
     achieved = alg_bytes / (ext_ms / 1000) / 1e9 if ext_ms > 0 else 0.0
     kernel_ms = {k: sum(s[k] for s in stats) / args.steps for k in ("extend_ms", "shadow_ms", "shade_ms", "other_ms")}

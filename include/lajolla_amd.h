@@ -269,8 +269,8 @@ typedef struct ljx_render_stats {
     uint64_t shadow_tri_tests;
     uint64_t instrumented_rays;
     double build_ms;               /* scene upload + BVH build of this scene */
-    int32_t bvh_depth;
-    int32_t pad;
+    int32_t bvh_depth;             /* nodes on the longest root-to-leaf path */
+    int32_t node_bytes;            /* bytes a traversal reads per node visit */
     double guide_ms;               /* HIP-event time in guide inference (guided renders) */
     double train_ms;               /* sample collection + Adam steps (guided renders) */
     uint64_t train_samples;        /* training samples recorded */

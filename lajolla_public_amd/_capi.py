@@ -200,7 +200,7 @@ class RenderStats(C.Structure):
         ("instrumented_rays", C.c_uint64),
         ("build_ms", C.c_double),
         ("bvh_depth", C.c_int32),
-        ("pad", C.c_int32),
+        ("node_bytes", C.c_int32),
         ("guide_ms", C.c_double),
         ("train_ms", C.c_double),
         ("train_samples", C.c_uint64),

@@ -2,7 +2,7 @@
 // centroids; subtrees above a size threshold are built on separate threads
 // into private node arrays and spliced, so the output is independent of the
 // thread count.  Deep ranges fall back to object-median splits so the tree
-// depth stays within the traversal stack (kMaxTreeDepth).
+// depth stays bounded (kMaxTreeDepth).
 #include "ljx_bvh.h"
 #include "ljx_trace.h"
 
@@ -186,6 +186,88 @@ struct Builder {
 
 }  // namespace
 
+namespace {
+
+struct Child {
+    int ref;
+    float lo[3], hi[3];
+    float half_area() const {
+        float d0 = hi[0] - lo[0], d1 = hi[1] - lo[1], d2 = hi[2] - lo[2];
+        return d0 * d1 + d1 * d2 + d2 * d0;
+    }
+};
+
+void binary_children(const BvhNode &nd, Child &a, Child &b) {
+    a.ref = nd.ch.x;
+    a.lo[0] = nd.lxy.x, a.hi[0] = nd.lxy.y, a.lo[1] = nd.lxy.z, a.hi[1] = nd.lxy.w, a.lo[2] = nd.zz.x, a.hi[2] = nd.zz.y;
+    b.ref = nd.ch.y;
+    b.lo[0] = nd.rxy.x, b.hi[0] = nd.rxy.y, b.lo[1] = nd.rxy.z, b.hi[1] = nd.rxy.w, b.lo[2] = nd.zz.z, b.hi[2] = nd.zz.w;
+}
+
+struct Collapser {
+    const std::vector<BvhNode> &bin;
+    std::vector<Bvh4Node> out;
+    int depth = 0, stack_need = 0;
+
+    // Emits the 4-wide node for binary node b; returns its index.  level and
+    // stacked: depth of this node and stack entries held on arrival.
+    int emit(int b, int level, int stacked) {
+        std::vector<Child> kids(2);
+        binary_children(bin[b], kids[0], kids[1]);
+        while (kids.size() < 4) {
+            int best = -1;
+            float area = -1;
+            for (size_t i = 0; i < kids.size(); i++)
+                if (kids[i].ref >= 0 && kids[i].half_area() > area) area = kids[i].half_area(), best = int(i);
+            if (best < 0) break;
+            Child a, c;
+            binary_children(bin[kids[best].ref], a, c);
+            kids[best] = a;
+            kids.insert(kids.begin() + best + 1, c);
+        }
+        int me = int(out.size());
+        out.emplace_back();
+        depth = std::max(depth, level);
+        // a traversal entering one child may have pushed all the others
+        int held = stacked + int(kids.size()) - 1;
+        stack_need = std::max(stack_need, held);
+        float lo[3][4], hi[3][4];
+        int ch[4];
+        for (int c = 0; c < 4; c++) {
+            for (int k = 0; k < 3; k++) lo[k][c] = kInf, hi[k][c] = -kInf;
+            ch[c] = kNoChild;
+        }
+        for (size_t c = 0; c < kids.size(); c++) {
+            for (int k = 0; k < 3; k++) lo[k][c] = kids[c].lo[k], hi[k][c] = kids[c].hi[k];
+            ch[c] = kids[c].ref >= 0 ? emit(kids[c].ref, level + 1, held) : kids[c].ref;
+        }
+        Bvh4Node &nd = out[me];
+        nd.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+        nd.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+        nd.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+        nd.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+        nd.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+        nd.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+        nd.ch = make_int4(ch[0], ch[1], ch[2], ch[3]);
+        nd.meta = make_int4(int(kids.size()), 0, 0, 0);
+        return me;
+    }
+};
+
+}  // namespace
+
+Bvh4Result collapse_bvh4(const BvhBuildResult &bin) {
+    Bvh4Result r;
+    if (bin.nodes.empty()) return r;
+    Collapser c{bin.nodes, {}, 0, 0};
+    c.out.reserve(bin.nodes.size() / 2 + 1);
+    c.emit(0, 1, 0);
+    r.nodes.swap(c.out);
+    r.depth = c.depth;
+    r.stack_need = c.stack_need;
+    return r;
+}
+
 BvhBuildResult build_bvh_sah(const std::vector<PrimBox> &boxes, int max_leaf, int threads) {
     BvhBuildResult res;
     if (boxes.empty()) return res;
@@ -208,7 +290,7 @@ BvhBuildResult build_bvh_sah(const std::vector<PrimBox> &boxes, int max_leaf, in
     res.nodes.swap(nodes);
     res.slot_prim = b.ids;
     res.max_depth = b.max_depth.load();
-    if (res.max_depth > kMaxTreeDepth) throw std::runtime_error("BVH deeper than the traversal stack");
+    if (res.max_depth > kMaxTreeDepth) throw std::runtime_error("BVH deeper than the build bound");
     return res;
 }
 

@@ -23,4 +23,15 @@ struct BvhBuildResult {
 // Builds over n primitive boxes.  max_leaf <= 8.  threads <= 0: all cores.
 BvhBuildResult build_bvh_sah(const std::vector<PrimBox> &boxes, int max_leaf, int threads);
 
+struct Bvh4Result {
+    std::vector<Bvh4Node> nodes;  // node 0 is the root
+    int depth = 0;                // nodes on the longest root-to-leaf path
+    int stack_need = 0;           // most stack entries a traversal can hold
+};
+
+// Collapses a binary BVH into a 4-wide one: each node takes its children's
+// largest-area inner nodes apart until it has four children or only leaves.
+// Leaves (primitive slot ranges) are unchanged.
+Bvh4Result collapse_bvh4(const BvhBuildResult &bin);
+
 }  // namespace ljx

@@ -166,6 +166,16 @@ struct CameraRec {  // src/camera.h:7-22
 //   zz  = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
 //   ch  = (child0, child1, -, -); child >= 0: inner node; child < 0: leaf
 //         ~child = (first_slot << 3) | (count - 1)
+// 4-wide node (128 bytes): the boxes of up to four children in SoA form
+// (child c in lane c of each float4), their references (inner node index >= 0,
+// leaf ~((first_slot << 3) | (count - 1)), kNoChild for an empty slot).
+constexpr int kNoChild = 0x7fffffff;
+struct alignas(16) Bvh4Node {
+    float4 lox, hix, loy, hiy, loz, hiz;
+    int4 ch;
+    int4 meta;  // meta.x: number of children
+};
+
 struct alignas(16) BvhNode {
     float4 lxy, rxy, zz;
     int4 ch;
@@ -173,7 +183,7 @@ struct alignas(16) BvhNode {
 
 struct DeviceScene {
     // traversal
-    const BvhNode *nodes;
+    const Bvh4Node *nodes;
     const float4 *prims;      // 3 float4 per leaf slot: v0|gid, v1|kind, v2|-
     const double4 *spheres;   // centre, radius per sphere primitive
     int num_nodes;

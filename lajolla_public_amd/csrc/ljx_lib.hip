@@ -131,6 +131,7 @@ struct Scene {
     double bound_radius = 0;
     float box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};  // axis-aligned bounds of every primitive
     int bvh_depth = 0;
+    int spill_entries = 1;  // per-lane global stack entries beyond the LDS ones
     double build_ms = 0;
     std::unique_ptr<RenderCtx> rc;
     int rc_alloc_W = 0, rc_alloc_H = 0;
@@ -376,7 +377,10 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
 
     // ---- BVH
     BvhBuildResult bvh = build_bvh_sah(boxes, 8, 0);
-    sc.bvh_depth = bvh.max_depth;
+    Bvh4Result bvh4 = collapse_bvh4(bvh);
+    sc.bvh_depth = bvh4.depth;
+    sc.spill_entries = std::max(1, bvh4.stack_need - kStackLds);
+    if (sc.spill_entries > kMaxStackSpill) fail(LJX_ERR_UNSUPPORTED, "BVH needs a deeper traversal stack than this build has");
     std::vector<float4> prims(bvh.slot_prim.size() * 3);
     for (size_t slot = 0; slot < bvh.slot_prim.size(); slot++) {
         int gid = bvh.slot_prim[slot];
@@ -471,8 +475,8 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
     S.camera = make_camera(d->camera);
 
     // ---- upload
-    S.nodes = sc.upload(bvh.nodes);
-    S.num_nodes = int(bvh.nodes.size());
+    S.nodes = sc.upload(bvh4.nodes);
+    S.num_nodes = int(bvh4.nodes.size());
     S.prims = sc.upload(prims);
     S.spheres = sc.upload(spheres);
     S.num_slots = int(bvh.slot_prim.size());
@@ -671,7 +675,7 @@ void ensure_ctx(Scene &sc, int capacity, int trace_grid) {
     P.sh_c = sc.alloc<float4>(C, rc->bufs);
     P.ctr = sc.alloc<int>(C_TOTAL, rc->bufs);
     P.stat = sc.alloc<unsigned long long>(S_NUM, rc->bufs);
-    P.spill = sc.alloc<int>(size_t(rc->trace_grid) * 256 * kStackSpill, rc->bufs);
+    P.spill = sc.alloc<int>(size_t(rc->trace_grid) * 256 * sc.spill_entries, rc->bufs);
     P.accum = sc.alloc<double>(size_t(sc.width) * sc.height * 3, rc->bufs);
     vol_alloc(sc.has_media, C, P, rc->bufs, [&](size_t bytes) {
         void *p = nullptr;
@@ -968,6 +972,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
         stats->instrumented_rays = stats_on ? st[S_EXT_RAYS] + st[S_SH_RAYS] : 0;
         stats->build_ms = sc.build_ms;
         stats->bvh_depth = sc.bvh_depth;
+        stats->node_bytes = 7 * 16;  // six SoA box vectors and the child references
         stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     } else {
         for (int k = 0; k < 6; k++) tm.total(k);
@@ -1267,7 +1272,7 @@ void trace_device(Scene &sc, const ljx_ray *rays, size_t n, void *out, const ljx
     }
     int grid = trace_grid_for(sc);
     SpillBuf spill;
-    HIPX(hipMalloc(&spill.p, size_t(grid) * 256 * kStackSpill * sizeof(int)));
+    HIPX(hipMalloc(&spill.p, size_t(grid) * 256 * sc.spill_entries * sizeof(int)));
     unsigned long long *stat = nullptr;
     HIPX(hipMalloc(&stat, sizeof(unsigned long long) * S_NUM));
     struct Free {
@@ -1306,6 +1311,7 @@ void trace_device(Scene &sc, const ljx_ray *rays, size_t n, void *out, const ljx
         }
         stats->build_ms = sc.build_ms;
         stats->bvh_depth = sc.bvh_depth;
+        stats->node_bytes = 7 * 16;  // six SoA box vectors and the child references
         stats->instrumented_rays = st_on ? n : 0;
     }
 }

@@ -1,12 +1,13 @@
 // BVH traversal for gfx950: closest hit and any hit, FP32, one ray per lane.
 //
 // Replaces Embree's rtcIntersect1 / rtcOccluded1 as lajolla calls them
-// (src/intersection.cpp:7-85).  Node layout: two child boxes per 64-byte node
-// (ljx_device.h BvhNode), loaded as four 16-byte vector loads.  The
-// traversal stack lives in LDS (STACK_LDS entries per lane, lane-interleaved
-// so a wave's pushes hit 64 consecutive banks) and spills to a per-lane
-// global area beyond that; the builder bounds the tree depth so the spill
-// area never overflows.  Triangles use the watertight test of Woop, Benthin
+// (src/intersection.cpp:7-85).  Node layout: a 4-wide BVH, the four child
+// boxes of a node in SoA form in 128 bytes (ljx_device.h Bvh4Node), loaded as
+// seven 16-byte vector loads; hit children are ordered near to far with a
+// five-comparator sorting network.  The traversal stack lives in LDS
+// (kStackLds entries per lane, lane-interleaved so a wave's pushes hit 64
+// consecutive banks) and spills to a per-lane global area beyond that, sized
+// per scene from the tree's worst-case stack need.  Triangles use the watertight test of Woop, Benthin
 // and Wald (JCGT 2013) with an FP64 re-test of edge hits.
 #pragma once
 
@@ -15,10 +16,10 @@
 namespace ljx {
 
 // 20 LDS entries x 256 lanes = 20 KB per block: eight traversal blocks fit
-// the 160 KB of LDS of a CU, i.e. eight waves per SIMD at 64 VGPRs.
-constexpr int kStackLds = 20;     // per-lane LDS stack entries
-constexpr int kStackSpill = 100;  // per-lane spill entries (tree depth <= kStackLds + kStackSpill)
-constexpr int kMaxTreeDepth = kStackLds + kStackSpill - 2;
+// the 160 KB of LDS of a CU, i.e. eight waves per SIMD.
+constexpr int kStackLds = 20;       // per-lane LDS stack entries
+constexpr int kMaxStackSpill = 256; // largest per-lane spill a scene may need
+constexpr int kMaxTreeDepth = 118;  // binary build depth bound (object-median splits below)
 constexpr int kPrimTri = 0, kPrimSphere = 1;
 
 struct TraceHit {
@@ -160,7 +161,7 @@ __device__ __forceinline__ bool sphere_roots(double ox, double oy, double oz, do
 }
 
 struct TraceScene {
-    const BvhNode *__restrict__ nodes;
+    const Bvh4Node *__restrict__ nodes;
     const float4 *__restrict__ prims;    // 3 per leaf slot
     const double4 *__restrict__ spheres; // centre xyz, radius
 };
@@ -172,35 +173,52 @@ template <bool ANY, bool STATS, bool SPH>
 __device__ __forceinline__ bool trace_ray(const TraceScene &ts, float ox, float oy, float oz, float dx, float dy,
                                           float dz, float tmin, float tmax, int *stack, int stride, int *spill,
                                           int spill_stride, TraceHit &hit, uint32_t &n_nodes, uint32_t &n_prims) {
-    const BvhNode *__restrict__ nodes = ts.nodes;
+    const Bvh4Node *__restrict__ nodes = ts.nodes;
     const float4 *__restrict__ prims = ts.prims;
+    const float kMiss = __builtin_huge_valf();
     RayPre r = ray_setup(ox, oy, oz, dx, dy, dz);
     int sp = 0;
-    int node = 0;  // root is always an inner node
+    int node = 0;  // the root is an inner node
     bool found = false;
     hit.slot = -1;
+    auto push = [&](int v) {
+        if (sp < kStackLds) stack[sp * stride] = v;
+        else spill[(sp - kStackLds) * spill_stride] = v;
+        sp++;
+    };
+    auto cswap = [](float &ta, int &ca, float &tb, int &cb) {
+        if (tb < ta) {
+            float t = ta;
+            ta = tb;
+            tb = t;
+            int c = ca;
+            ca = cb;
+            cb = c;
+        }
+    };
     while (true) {
         if (node >= 0) {
-            const BvhNode *nd = nodes + node;
-            float4 a = nd->lxy, b = nd->rxy, z = nd->zz;
+            const Bvh4Node *nd = nodes + node;
+            float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
             int4 ch = nd->ch;
             if (STATS) n_nodes++;
-            float e0 = box_entry(r, a.x, a.y, a.z, a.w, z.x, z.y, tmin, tmax);
-            float e1 = box_entry(r, b.x, b.y, b.z, b.w, z.z, z.w, tmin, tmax);
-            bool h0 = e0 != __builtin_huge_valf(), h1 = e1 != __builtin_huge_valf();
-            if (h0 && h1) {
-                int nearc = ch.x, farc = ch.y;
-                if (e1 < e0) {
-                    nearc = ch.y;
-                    farc = ch.x;
-                }
-                if (sp < kStackLds) stack[sp * stride] = farc;
-                else spill[(sp - kStackLds) * spill_stride] = farc;
-                sp++;
-                node = nearc;
-                continue;
-            } else if (h0 || h1) {
-                node = h0 ? ch.x : ch.y;
+            float t0 = box_entry(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tmin, tmax);
+            float t1 = box_entry(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tmin, tmax);
+            float t2 = box_entry(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tmin, tmax);
+            float t3 = box_entry(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tmin, tmax);
+            int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+            if (c2 == kNoChild) t2 = kMiss;  // empty slots carry inverted boxes, which
+            if (c3 == kNoChild) t3 = kMiss;  // a slab test alone does not reject
+            cswap(t0, c0, t1, c1);
+            cswap(t2, c2, t3, c3);
+            cswap(t0, c0, t2, c2);
+            cswap(t1, c1, t3, c3);
+            cswap(t1, c1, t2, c2);
+            if (t0 != kMiss) {
+                if (t3 != kMiss) push(c3);
+                if (t2 != kMiss) push(c2);
+                if (t1 != kMiss) push(c1);
+                node = c0;
                 continue;
             }
         } else {
