@@ -38,9 +38,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 96-job batched-rendering run")
     ap.add_argument("--guide-ref-spp", type=int, default=2048,
                     help="spp of the plain reference image the relMSE figures are measured against")
-    ap.add_argument("--mode", choices=["render", "copycal"], default="render",
-                    help="copycal: one plain FP32 elementwise kernel over 1 GiB (read) + 1 GiB (write), "
-                         "to scale the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--mode", choices=["render", "guided", "copycal"], default="render",
+                    help="guided: guided renders of the guiding workload (terrain_gap 1080p, 64 spp, an untrained "
+                         "network per step), with the matrix-core roofline of the inference kernel; copycal: one "
+                         "plain FP32 elementwise kernel over 1 GiB (read) + 1 GiB (write), to scale the FETCH_SIZE / "
+                         "WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); every box here has one GPU.",
@@ -61,6 +63,10 @@ def main():
     import numpy as np
     from lajolla_public_amd import _capi as capi
     from lajolla_public_amd import api, scenes
+
+    if args.mode == "guided":
+        guided_mode(args)
+        return
 
     t0 = time.time()
     if args.workload == "terrain_spheres":
@@ -180,6 +186,69 @@ def main():
     }
     if cpu and cpu.get("value"):
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
+GUIDE_FLOP_PER_QUERY = 2 * (160 * 64 + 3 * 64 * 64 + 64 * 40)  # 50,176
+GUIDE_TRAIN_FLOP_PER_SAMPLE = 3 * GUIDE_FLOP_PER_QUERY  # forward, input deltas, weight gradients
+
+
+def guided_mode(args):
+    """Guided renders of terrain_gap at 1080p, 64 spp, depth 8, each step from
+    an untrained network (8 recording passes, 6 Adam steps on 2^20-sample
+    batches after each).  Roofline of k_guide_infer: 50,176 FLOP per query
+    (one query per extension-queue entry, the whole wave runs the MLP)."""
+    from lajolla_public_amd import api, scenes
+    desc = scenes.terrain_gap(seed=7, spp=args.spp or 64)
+    sc = api.Scene(desc)
+    W, H, spp = desc.camera["width"], desc.camera["height"], desc.options["samples_per_pixel"]
+    opts = scenes.render_options(**desc.options)
+    ex = api.exec_options(collect_timing=1)
+    go = api.guide_options(mix_prob=0.5, train_spp=8, steps_per_pass=6)
+
+    def step(seed):
+        g = api.Guide(seed=seed)
+        t = time.perf_counter()
+        _, st = api.render_guided(sc, g, opts, exec_options=ex, gopts=go, stats=True)
+        ms = (time.perf_counter() - t) * 1000
+        g.close()
+        return ms, st
+
+    for k in range(args.warmup):
+        step(1000 + k)
+    runs = [step(k + 1) for k in range(args.steps)]
+    ms = sum(r[0] for r in runs) / len(runs)
+    st = [r[1] for r in runs]
+    ext = sum(s["extension_rays"] for s in st)
+    g_ms = sum(s["guide_ms"] for s in st)
+    t_ms = sum(s["train_ms"] for s in st)
+    steps = sum(s["train_steps"] for s in st)
+    achieved = ext * GUIDE_FLOP_PER_QUERY / (g_ms / 1000) / 1e12
+    line = {
+        "metric": "Msamples/s (1080p, depth 8, guided, training included)",
+        "value": round(W * H * spp / (ms / 1000) / 1e6, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 MFMA (fp32 accumulate) network; f32 traversal / f64 shading",
+        "data": "synthetic (seeded scene generator, random-init network)",
+        "config": {"workload": "terrain_gap 1920x1080, 64 spp, depth 8, guided (mix 0.5), 8 training passes x 6 "
+                               "Adam steps", "width": W, "height": H, "spp": spp},
+        "kernel_ms_per_step": {k: round(sum(s[k] for s in st) / len(st), 2)
+                               for k in ("extend_ms", "shadow_ms", "shade_ms", "guide_ms", "train_ms", "other_ms")},
+        "roofline": {
+            "kernel": "k_guide_infer (MLP forward + lobes + sampling)", "bound": "mfma",
+            "achieved": round(achieved, 1), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
+            "flop_per_query": GUIDE_FLOP_PER_QUERY, "queries_per_step": ext // len(st),
+        },
+        "training": {"adam_steps_per_step": steps // len(st),
+                     "samples_per_step": sum(s["train_step_samples"] for s in st) // len(st),
+                     "achieved_tflops_incl_collect": round(
+                         sum(s["train_step_samples"] for s in st) * GUIDE_TRAIN_FLOP_PER_SAMPLE / (t_ms / 1000) / 1e12,
+                         1) if t_ms > 0 else None},
+    }
     print(json.dumps(line), flush=True)
 
 

@@ -275,6 +275,7 @@ typedef struct ljx_render_stats {
     double train_ms;               /* sample collection + Adam steps (guided renders) */
     uint64_t train_samples;        /* training samples recorded */
     uint64_t train_steps;          /* Adam steps taken */
+    uint64_t train_step_samples;   /* samples summed over the Adam steps' batches */
 } ljx_render_stats;
 
 /* Render-call switches that do not change results. */

@@ -205,6 +205,7 @@ class RenderStats(C.Structure):
         ("train_ms", C.c_double),
         ("train_samples", C.c_uint64),
         ("train_steps", C.c_uint64),
+        ("train_step_samples", C.c_uint64),
     ]
 
 

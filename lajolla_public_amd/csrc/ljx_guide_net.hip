@@ -284,14 +284,26 @@ __global__ __launch_bounds__(256) void k_wgrad(const float4 *feats, int n, const
     }
 }
 
-// Sums the partial gradients, normalises by the batch's weight sum and, with
-// update set, takes one Adam step (Kingma and Ba 2015, bias-corrected).
-__global__ void k_adam(const float *partial, int nblk, const float *red, float *master, float *m, float *v,
-                       float *grad, float lr, float b1, float b2, float eps, float bc1, float bc2, int update) {
+// First stage of the partial-gradient sum: slice blockIdx.y of kReduceSlices
+// adds its share of the nblk per-block rows (in a fixed order).
+constexpr int kReduceSlices = 32;
+__global__ void k_reduce_partials(const float *partial, int nblk, float *out) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kNumParams) return;
     float s = 0.f;
-    for (int b = 0; b < nblk; b++) s += partial[size_t(b) * kNumParams + p];
+    for (int b = blockIdx.y; b < nblk; b += kReduceSlices) s += partial[size_t(b) * kNumParams + p];
+    out[size_t(blockIdx.y) * kNumParams + p] = s;
+}
+
+// Sums the sliced partial gradients, normalises by the batch's weight sum
+// and, with update set, takes one Adam step (Kingma and Ba 2015,
+// bias-corrected).
+__global__ void k_adam(const float *sliced, const float *red, float *master, float *m, float *v, float *grad,
+                       float lr, float b1, float b2, float eps, float bc1, float bc2, int update) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kNumParams) return;
+    float s = 0.f;
+    for (int b = 0; b < kReduceSlices; b++) s += sliced[size_t(b) * kNumParams + p];
     float ws = red[1];
     float gp = ws > 0.f ? s / ws : 0.f;
     grad[p] = gp;
@@ -421,7 +433,7 @@ Net *net_create(int device, const ljx_guide_config &cfg) {
         n->batch_cap = (cfg.batch_size + 31) / 32 * 32;
         dalloc(n->acts, act_elems(n->batch_cap / 32));
         n->partial_blocks = 2 * n->num_cu;
-        dalloc(n->partial, size_t(n->partial_blocks) * kNumParams);
+        dalloc(n->partial, size_t(n->partial_blocks + kReduceSlices) * kNumParams);
         std::vector<float> p = init_params(cfg.seed);
         net_set_params(*n, p.data());
     } catch (...) {
@@ -480,9 +492,12 @@ void net_step(Net &n, const float4 *feats, const float4 *dirs, int count, bool u
         bc1 = float(1.0 - std::pow(double(n.cfg.beta1), n.steps));
         bc2 = float(1.0 - std::pow(double(n.cfg.beta2), n.steps));
     }
-    hipLaunchKernelGGL(k_adam, dim3((kNumParams + 255) / 256), dim3(256), 0, s, n.partial, nblk, n.red, n.master,
-                       n.adam_m, n.adam_v, n.grad, n.cfg.learning_rate, n.cfg.beta1, n.cfg.beta2, n.cfg.epsilon, bc1,
-                       bc2, update ? 1 : 0);
+    float *sliced = n.partial + size_t(n.partial_blocks) * kNumParams;
+    hipLaunchKernelGGL(k_reduce_partials, dim3((kNumParams + 255) / 256, kReduceSlices), dim3(256), 0, s, n.partial,
+                       nblk, sliced);
+    hipLaunchKernelGGL(k_adam, dim3((kNumParams + 255) / 256), dim3(256), 0, s, sliced, n.red, n.master, n.adam_m,
+                       n.adam_v, n.grad, n.cfg.learning_rate, n.cfg.beta1, n.cfg.beta2, n.cfg.epsilon, bc1, bc2,
+                       update ? 1 : 0);
     HIPX(hipGetLastError());
     if (update) repack(n, s);
     if (loss_out) {

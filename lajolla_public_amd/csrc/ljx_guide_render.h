@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(DeviceScene S, PathBufs P, 
                     int b = P.depth[pid];
                     guide_draws(P, pid, b, su[w][lane]);
                     if (b < P.rec_depth) {
-                        size_t r = size_t(b) * P.capacity + pid;
+                        size_t r = size_t(b) * P.rec_stride + pid;
                         P.rec_feat[2 * r] = make_float4(f[0], f[1], f[2], f[3]);
                         P.rec_feat[2 * r + 1] = make_float4(f[4], f[5], f[6], 0.f);
                     }
@@ -164,13 +164,15 @@ __global__ void k_guide_collect(PathBufs P, float4 *out_feat, float4 *out_dir, i
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         int i = base + threadIdx.x;
         bool live = i < n;
+        // vertices past the one the path ended at hold nothing of this pass
+        const int last = live ? P.depth[i] : -1;
         float sx = 0.f, sy = 0.f, sz = 0.f;
         for (int b = R - 1; b >= 0; b--) {
-            size_t r = size_t(b) * P.capacity + (live ? i : 0);
+            size_t r = size_t(b) * P.rec_stride + (live ? i : 0);
             bool take = false;
             float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
             float wgt = 0.f;
-            if (live) {
+            if (live && b <= last) {
                 d = P.rec_dir[r];
                 if (d.w > 0.5f) {
                     float4 be = P.rec_beta[r];

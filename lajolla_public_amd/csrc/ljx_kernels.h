@@ -88,6 +88,7 @@ struct PathBufs {
     float4 *rec_beta;  // rec_depth x capacity: throughput at the vertex
     float4 *rec_c;     // rec_depth x capacity: radiance the vertex adds (emission, NEE, environment)
     int rec_depth;     // vertices recorded per path (0: not recording)
+    int rec_stride;    // record arrays are [vertex][slot]: the pass's slot count
     uint32_t guide_mask;  // material types sampled from the mixture
     float guide_mix;   // probability of the guide in the mixture
     float box_lo[3], box_scale[3];  // position features: (p - lo) * scale
@@ -285,7 +286,7 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
         lobes = P.glob + size_t(pid) * guide::kLobes;
     }
     const bool rec = GUIDE && b < P.rec_depth;
-    const size_t rix = size_t(b) * P.capacity + pid;
+    const size_t rix = size_t(b) * P.rec_stride + pid;
     D3 rad_in = rad;
     if (b == 0) {
         // path_tracing.h:58-61
@@ -473,7 +474,7 @@ __global__ __launch_bounds__(256) void k_miss(DeviceScene S, PathBufs P) {
         P.rad[pid] = make_float4(float(rad.x), float(rad.y), float(rad.z), 0.f);
         if (b < P.rec_depth) {
             D3 add = rad - rad_in;
-            P.rec_c[size_t(b) * P.capacity + pid] = make_float4(float(add.x), float(add.y), float(add.z), 0.f);
+            P.rec_c[size_t(b) * P.rec_stride + pid] = make_float4(float(add.x), float(add.y), float(add.z), 0.f);
         }
     }
 }
@@ -504,7 +505,7 @@ __global__ __launch_bounds__(256) void k_shadow(DeviceScene S, PathBufs P) {
             P.rad[pid] = make_float4(r.x + c.x, r.y + c.y, r.z + c.z, 0.f);
             int b = int(bits >> kSlotBits);
             if (b < P.rec_depth) {
-                float4 *rc = &P.rec_c[size_t(b) * P.capacity + pid];
+                float4 *rc = &P.rec_c[size_t(b) * P.rec_stride + pid];
                 float4 o = *rc;
                 *rc = make_float4(o.x + c.x, o.y + c.y, o.z + c.z, 0.f);
             }

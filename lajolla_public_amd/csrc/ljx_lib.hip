@@ -867,7 +867,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
     HIPX(hipGetLastError());
     tm.end(3, ev);
 
-    uint64_t passes = 0, train_samples = 0, train_steps = 0;
+    uint64_t passes = 0, train_samples = 0, train_steps = 0, train_step_samples = 0;
     int max_bounces = opt.max_depth >= 0 ? std::max(opt.max_depth, 1) : 1 << 20;
     std::vector<int> host_ctr(C_TOTAL);
     const int train_spp = guided ? std::min(gopt.train_spp, spp) : 0;
@@ -882,6 +882,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
             P.pass_ns = ns;
             P.pass_q0 = q0;
             P.pass_nq = nq;
+            P.rec_stride = ns * nq;
             passes++;
             tm.begin(3, ev);
             launch(k_reset_counters, 1, 64, s, P.ctr, 0, C_TOTAL);
@@ -889,7 +890,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
             if (vol) launch(k_vol_init, shade_grid, 256, s, S, P);
             if (recording)
                 launch(k_guide_clear, shade_grid, 256, s, P.rec_dir, P.rec_beta, P.rec_c,
-                       size_t(P.rec_depth) * P.capacity);
+                       size_t(P.rec_depth) * P.rec_stride);
             tm.end(3, ev);
             int cur = 0;
             for (int b = 0; b < max_bounces; b++) {
@@ -935,6 +936,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
                     int cntk = int(std::min<long long>(net->batch_cap, navail - start));
                     guide::net_step(*net, gc->s_feat + 2 * start, gc->s_dir + start, cntk, true, s, nullptr);
                     train_steps++;
+                    train_step_samples += uint64_t(cntk);
                 }
                 tm.end(5, ev);
             }
@@ -965,6 +967,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
         stats->train_ms = tm.total(5);
         stats->train_samples = train_samples;
         stats->train_steps = train_steps;
+        stats->train_step_samples = train_step_samples;
         stats->node_visits = st[S_NODES];
         stats->tri_tests = st[S_PRIMS];
         stats->shadow_node_visits = st[S_SH_NODES];
