@@ -209,11 +209,19 @@ __device__ __forceinline__ bool trace_ray(const TraceScene &ts, float ox, float 
             int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
             if (c2 == kNoChild) t2 = kMiss;  // empty slots carry inverted boxes, which
             if (c3 == kNoChild) t3 = kMiss;  // a slab test alone does not reject
-            cswap(t0, c0, t1, c1);
-            cswap(t2, c2, t3, c3);
-            cswap(t0, c0, t2, c2);
-            cswap(t1, c1, t3, c3);
-            cswap(t1, c1, t2, c2);
+            if (ANY) {
+                // any hit: only the first child to visit matters (the nearest);
+                // the others are pushed in whatever order, each if it was hit
+                cswap(t0, c0, t1, c1);
+                cswap(t2, c2, t3, c3);
+                cswap(t0, c0, t2, c2);
+            } else {
+                cswap(t0, c0, t1, c1);
+                cswap(t2, c2, t3, c3);
+                cswap(t0, c0, t2, c2);
+                cswap(t1, c1, t3, c3);
+                cswap(t1, c1, t2, c2);
+            }
             if (t0 != kMiss) {
                 if (t3 != kMiss) push(c3);
                 if (t2 != kMiss) push(c2);
