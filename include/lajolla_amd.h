@@ -324,8 +324,11 @@ int ljx_scene_destroy(void *scene);
 int ljx_scene_bounds(void *scene, double *center_radius4);
 
 /* Renders into rgb_out (host memory, height*width*3 floats, HWC, the pixel
- * mean over samples_per_pixel).  Replaces render() (src/render.cpp:155-170).
- * opts may be NULL (scene options are used); exec and stats may be NULL. */
+ * mean over samples_per_pixel).  Replaces render() (src/render.cpp:155-170);
+ * the auxiliary integrators (depth ... mipmap level) take one pixel-centre
+ * ray per pixel and ignore samples_per_pixel, as aux_render
+ * (src/render.cpp:12-69).  opts may be NULL (scene options are used); exec
+ * and stats may be NULL. */
 int ljx_render(void *scene, const ljx_render_options *opts,
                const ljx_exec_options *exec, float *rgb_out,
                ljx_render_stats *stats);

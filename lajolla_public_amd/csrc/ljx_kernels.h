@@ -570,6 +570,49 @@ __global__ void k_job_resolve(const BatchJob *jobs, int j) {
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) J.out[i] = float(J.accum[i] * inv);
 }
 
+// Auxiliary integrators (src/render.cpp:12-69): one ray through each pixel
+// centre; depth, shading normal, mean curvature, the camera ray
+// differential (radius, spread) or the mipmap level (none with constant
+// textures: black).  Misses are black.
+template <bool SPH>
+__global__ __launch_bounds__(256) void k_aux(DeviceScene S, PathBufs P, int mode, float *out) {
+    __shared__ int stack[kStackLds * 256];
+    const int n = P.W * P.H;
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    int *spill = P.spill + gtid;
+    uint32_t nn = 0, np = 0;
+    for (int i = gtid; i < n; i += stride) {
+        int x = i % P.W, y = i / P.W;
+        D3 o, d;
+        sample_primary(S.camera, (x + 0.5) / P.W, (y + 0.5) / P.H, o, d);
+        TraceHit h;
+        bool hit = S.num_nodes > 0 &&
+                   trace_ray<false, false, SPH>(TraceScene{S.nodes, S.prims, S.spheres}, float(o.x), float(o.y),
+                                                float(o.z), float(d.x), float(d.y), float(d.z), 0.f,
+                                                __builtin_huge_valf(), stack + threadIdx.x, blockDim.x, spill, stride,
+                                                h, nn, np);
+        D3 c{0, 0, 0};
+        if (hit) {
+            Vertex v = make_vertex<SPH>(S, h.slot, h.t, h.u, h.v, o, d);
+            if (mode == 0) {
+                double dist = sqrt(dist2(v.pos, o));
+                c = D3{dist, dist, dist};
+            } else if (mode == 1) {
+                c = v.g.sf.n;
+            } else if (mode == 2) {
+                double k = vertex_mean_curvature(S, h.slot, double(h.u), double(h.v), v);
+                c = D3{k, k, k};
+            } else if (mode == 3) {
+                c = D3{0, 0.25 / double(max(P.W, P.H)), 0};  // init_ray_differential (src/ray.h:35-37)
+            }
+        }
+        out[3 * size_t(i)] = float(c.x);
+        out[3 * size_t(i) + 1] = float(c.y);
+        out[3 * size_t(i) + 2] = float(c.z);
+    }
+}
+
 __global__ void k_resolve(const double *accum, float *out, size_t n, double inv_spp) {
     size_t stride = size_t(gridDim.x) * blockDim.x;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)

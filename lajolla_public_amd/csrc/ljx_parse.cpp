@@ -2,8 +2,9 @@
 // (src/parsers/parse_scene.cpp) and Wavefront OBJ meshes
 // (src/parsers/parse_obj.cpp), turned into an ljx_scene_desc.
 //
-// Covered: <default>, integrators path / volpath / direct (and the debug
-// integrators, which the device rejects at render time), the perspective
+// Covered: <default>, integrators path / volpath / direct and the auxiliary
+// views (depth, shading normal, mean curvature, ray differential, mipmap
+// level), the perspective
 // sensor with fovAxis, film and reconstruction filter, sampler sampleCount,
 // every BSDF of the reference with constant textures, obj / sphere /
 // rectangle shapes, area / point / directional emitters, homogeneous media,

@@ -115,6 +115,34 @@ __device__ __forceinline__ Vertex make_vertex(const DeviceScene &S, int slot, fl
     return vx;
 }
 
+// Mean curvature of a hit (compute_shading_info, src/shapes/triangle_mesh.inl:
+// 133-160 and src/shapes/sphere.inl:264): 1/r on spheres, 0 on meshes without
+// vertex normals, otherwise (dn/du . tangent + dn/dv . bitangent) / 2 with
+// the parameter derivatives written as the reference writes them.
+__device__ inline double vertex_mean_curvature(const DeviceScene &S, int slot, double s0, double s1,
+                                               const Vertex &vx) {
+    const ShapeRec &sh = S.shapes[vx.shape_id];
+    if (sh.type == 0) return 1.0 / sh.radius;
+    if (!sh.has_normals) return 0.0;
+    int gid = __float_as_int(S.prims[3 * slot].w);
+    int i0 = S.tri_vidx[3 * gid], i1 = S.tri_vidx[3 * gid + 1], i2 = S.tri_vidx[3 * gid + 2];
+    double u0x = 0, u0y = 0, u1x = 1, u1y = 0, u2x = 1, u2y = 1;
+    if (sh.has_uvs) {
+        u0x = S.vuv[2 * i0], u0y = S.vuv[2 * i0 + 1];
+        u1x = S.vuv[2 * i1], u1y = S.vuv[2 * i1 + 1];
+        u2x = S.vuv[2 * i2], u2y = S.vuv[2 * i2 + 1];
+    }
+    double dsx = u2x - u0x, dsy = u2y - u0y, dtx = u2x - u1x, dty = u2y - u1y;
+    double det = dsx * dty - dtx * dsy;
+    double dsdu = dty / det, dtdu = -dsy / det, dsdv = dtx / det, dtdv = -dsx / det;
+    D3 n0{S.vnrm[3 * i0], S.vnrm[3 * i0 + 1], S.vnrm[3 * i0 + 2]};
+    D3 n1{S.vnrm[3 * i1], S.vnrm[3 * i1 + 1], S.vnrm[3 * i1 + 2]};
+    D3 n2{S.vnrm[3 * i2], S.vnrm[3 * i2 + 1], S.vnrm[3 * i2 + 2]};
+    D3 dnds = n2 - n0, dndt = n2 - n1;
+    D3 dndu = dnds * dsdu + dndt * dtdu, dndv = dnds * dsdv + dndt * dtdv;
+    return (dot(dndu, vx.g.sf.x) + dot(dndv, vx.g.sf.y)) / 2;
+}
+
 // ------------------------------------------------------------------ lights
 struct LightPoint {
     D3 pos, n;

@@ -85,8 +85,17 @@ __device__ __forceinline__ float box_entry(const RayPre &r, float lx, float hx, 
     return tn <= tf ? tn : __builtin_huge_valf();
 }
 
-// Watertight ray/triangle test.  On a hit inside (tmin, tmax] writes t and the
-// barycentrics of v1 (u) and v2 (v).
+// a * b - c * d with both products rounded.  The fused form the compiler
+// would make (one product exact, the other rounded) is not antisymmetric:
+// for an edge two triangles share it gives both the same small nonzero value
+// instead of exact negatives, and a ray through the edge misses both.
+__device__ __forceinline__ float edge_fn(float a, float b, float c, float d) {
+#pragma clang fp contract(off)
+    return a * b - c * d;
+}
+
+// Watertight ray/triangle test (Woop, Benthin and Wald 2013).  On a hit
+// inside (tmin, tmax] writes t and the barycentrics of v1 (u) and v2 (v).
 __device__ __forceinline__ bool tri_test(const RayPre &r, float4 a4, float4 b4, float4 c4, float tmin, float tmax,
                                          float &t_out, float &u_out, float &v_out) {
     float Ax0 = a4.x - r.ox, Ay0 = a4.y - r.oy, Az0 = a4.z - r.oz;
@@ -99,9 +108,9 @@ __device__ __forceinline__ bool tri_test(const RayPre &r, float4 a4, float4 b4, 
     float By = sel3(Bx0, By0, Bz0, r.ky) - r.Sy * Bkz;
     float Cx = sel3(Cx0, Cy0, Cz0, r.kx) - r.Sx * Ckz;
     float Cy = sel3(Cx0, Cy0, Cz0, r.ky) - r.Sy * Ckz;
-    float U = Cx * By - Cy * Bx;
-    float V = Ax * Cy - Ay * Cx;
-    float W = Bx * Ay - By * Ax;
+    float U = edge_fn(Cx, By, Cy, Bx);
+    float V = edge_fn(Ax, Cy, Ay, Cx);
+    float W = edge_fn(Bx, Ay, By, Ax);
     if (U == 0.0f || V == 0.0f || W == 0.0f) {
         // edge case: redo the 2D edge functions in FP64 for watertightness
         U = float(double(Cx) * double(By) - double(Cy) * double(Bx));

@@ -54,6 +54,7 @@ struct PathVertex {  // intersection.h:15-35
     V3 position, geometric_normal;
     Frame shading_frame;
     V2 st, uv;
+    Real mean_curvature = 0;
     int shape_id = -1, primitive_id = -1, material_id = -1;
     int interior_medium_id = -1, exterior_medium_id = -1;
 };
@@ -99,29 +100,40 @@ struct HitRec {
     V3 ng;
     Real min_bary = 0;
 };
-// Moller-Trumbore in FP64 on the FP32 vertices; u weights v1 and v weights v2.
+// The exact closest hit, as Embree's intersector reports it, includes a ray
+// through an edge two triangles share.  Moller-Trumbore, even in FP64, can
+// drop such a ray (each triangle rounds its own barycentric just outside), so
+// the test here is the watertight one of Woop, Benthin and Wald (2013) in
+// FP64 on the FP32 vertices and ray: the 2D edge functions of a shared edge
+// are exact negatives in the two triangles, and a ray through it hits one of
+// them.  u weights v1 and v weights v2.
 inline bool tri_hit(const Shape &sh, int tri, const Ray &r, Real tmin, Real tmax, HitRec &h) {
     const V3 &p0 = sh.positions[sh.indices[3 * tri]];
     const V3 &p1 = sh.positions[sh.indices[3 * tri + 1]];
     const V3 &p2 = sh.positions[sh.indices[3 * tri + 2]];
-    V3 e1 = p1 - p0, e2 = p2 - p0;
-    V3 pv = cross(r.dir, e2);
-    Real det = dot(e1, pv);
+    const Real d[3] = {r.dir.x, r.dir.y, r.dir.z};
+    const Real ad[3] = {std::fabs(d[0]), std::fabs(d[1]), std::fabs(d[2])};
+    int kz = ad[0] > ad[1] ? (ad[0] > ad[2] ? 0 : 2) : (ad[1] > ad[2] ? 1 : 2);
+    int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+    if (d[kz] < 0) std::swap(kx, ky);
+    const Real Sx = d[kx] / d[kz], Sy = d[ky] / d[kz], Sz = 1 / d[kz];
+    const V3 a = p0 - r.org, b = p1 - r.org, c = p2 - r.org;
+    auto comp = [](const V3 &v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); };
+    const Real Ax = comp(a, kx) - Sx * comp(a, kz), Ay = comp(a, ky) - Sy * comp(a, kz);
+    const Real Bx = comp(b, kx) - Sx * comp(b, kz), By = comp(b, ky) - Sy * comp(b, kz);
+    const Real Cx = comp(c, kx) - Sx * comp(c, kz), Cy = comp(c, ky) - Sy * comp(c, kz);
+    const Real U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if ((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0)) return false;
+    const Real det = U + V + W;
     if (det == 0) return false;
-    Real inv = 1 / det;
-    V3 tv = r.org - p0;
-    Real u = dot(tv, pv) * inv;
-    if (u < 0 || u > 1) return false;
-    V3 qv = cross(tv, e1);
-    Real v = dot(r.dir, qv) * inv;
-    if (v < 0 || u + v > 1) return false;
-    Real t = dot(e2, qv) * inv;
+    const Real T = Sz * (U * comp(a, kz) + V * comp(b, kz) + W * comp(c, kz));
+    const Real t = T / det;
     if (!(t >= tmin && t <= tmax)) return false;
     h.t = t;
-    h.u = u;
-    h.v = v;
-    h.ng = cross(e1, e2);
-    h.min_bary = std::min(std::min(u, v), 1 - u - v);
+    h.u = V / det;
+    h.v = W / det;
+    h.ng = cross(p1 - p0, p2 - p0);
+    h.min_bary = std::min(std::min(h.u, h.v), U / det);
     return true;
 }
 // sphere.inl:12-38, 40-106 (numerically stable quadratic; st from azimuth/elevation)
@@ -400,6 +412,7 @@ static void shading_info(const Shape &sh, PathVertex &v) {
         V3 tangent = normalize(dpdu - v.geometric_normal * dot(v.geometric_normal, dpdu));
         v.shading_frame = Frame(tangent, normalize(cross(v.geometric_normal, tangent)), v.geometric_normal);
         v.uv = v.st;
+        v.mean_curvature = 1 / r;
         return;
     }
     int i0 = sh.indices[3 * v.primitive_id], i1 = sh.indices[3 * v.primitive_id + 1],
@@ -422,6 +435,7 @@ static void shading_info(const Shape &sh, PathVertex &v) {
     V2 duvdt{uvs[2].x - uvs[1].x, uvs[2].y - uvs[1].y};
     Real det = duvds.x * duvdt.y - duvdt.x * duvds.y;
     Real dsdu = duvdt.y / det, dtdu = -duvds.y / det;
+    Real dsdv = duvdt.x / det, dtdv = -duvds.x / det;
     V3 dpdu;
     if (std::fabs(det) > 1e-8f) {
         V3 dpds = p2 - p0, dpdt = p2 - p1;
@@ -436,6 +450,12 @@ static void shading_info(const Shape &sh, PathVertex &v) {
     V3 tangent = normalize(dpdu - n * dot(n, dpdu));
     V3 bitangent = normalize(cross(n, tangent));
     v.shading_frame = Frame(tangent, bitangent, n);
+    if (!sh.normals.empty()) {
+        // mean curvature from the normal's derivatives along the frame
+        V3 dnds = sh.normals[i2] - sh.normals[i0], dndt = sh.normals[i2] - sh.normals[i1];
+        V3 dndu = dnds * dsdu + dndt * dtdu, dndv = dnds * dsdv + dndt * dtdv;
+        v.mean_curvature = (dot(dndu, tangent) + dot(dndv, bitangent)) / 2;
+    }
 }
 
 // intersection.cpp:7-65
@@ -1254,6 +1274,34 @@ int orc_render(void *sp, const ljx_render_options *opts, int plan, int threads, 
     if (tile_end <= 0 || tile_end > ntiles) tile_end = ntiles;
     if (tile_begin < 0) tile_begin = 0;
     bool vol = s->options.integrator == LJX_INTEGRATOR_VOLPATH;
+    if (s->options.integrator <= LJX_INTEGRATOR_MIPMAP_LEVEL) {
+        // render.cpp:12-69: one ray through each pixel centre
+        const int mode = int(s->options.integrator);
+        parallel_for(size_t(h), threads, [&](size_t yy) {
+            int y = int(yy);
+            for (int x = 0; x < w; x++) {
+                Ray ray = sample_primary(s->camera, V2{(x + 0.5) / w, (y + 0.5) / h});
+                PathVertex v;
+                V3 c;
+                if (intersect(*s, ray, v, nullptr)) {
+                    Real dist = length(v.position - ray.org);
+                    if (mode == LJX_INTEGRATOR_DEPTH) c = V3{dist, dist, dist};
+                    else if (mode == LJX_INTEGRATOR_SHADING_NORMAL) c = v.shading_frame.n;
+                    else if (mode == LJX_INTEGRATOR_MEAN_CURVATURE)
+                        c = V3{v.mean_curvature, v.mean_curvature, v.mean_curvature};
+                    else if (mode == LJX_INTEGRATOR_RAY_DIFFERENTIAL)
+                        c = V3{0, Real(0.25) / std::max(w, h), 0};  // the camera's differential (ray.h:35-37)
+                    // LJX_INTEGRATOR_MIPMAP_LEVEL: constant textures have no level: black
+                }
+                size_t o = (size_t(y) * w + x) * 3;
+                rgb_out[o] = float(c.x);
+                rgb_out[o + 1] = float(c.y);
+                rgb_out[o + 2] = float(c.z);
+            }
+        });
+        if (counters) counters[0] = counters[1] = uint64_t(w) * h, counters[2] = 0;
+        return LJX_OK;
+    }
     if (s->options.integrator != LJX_INTEGRATOR_PATH && !vol) {
         g_err = "oracle renders the path and volpath integrators only";
         return LJX_ERR_UNSUPPORTED;
