@@ -75,6 +75,9 @@ enum { LJX_LIGHT_DIFFUSE_AREA = 0, LJX_LIGHT_ENVMAP = 1 };
 /* Medium = variant<HomogeneousMedium, HeterogeneousMedium> src/medium.h:22 */
 enum { LJX_MEDIUM_HOMOGENEOUS = 0, LJX_MEDIUM_HETEROGENEOUS = 1 };
 
+/* Volume<T> = variant<ConstantVolume<T>, GridVolume<T>>   src/volume.h:13-31 */
+enum { LJX_VOLUME_CONSTANT = 0, LJX_VOLUME_GRID = 1 };
+
 /* PhaseFunction = variant<IsotropicPhase, HenyeyGreenstein> src/phase_function.h:16 */
 enum { LJX_PHASE_ISOTROPIC = 0, LJX_PHASE_HENYEY_GREENSTEIN = 1 };
 
@@ -156,13 +159,37 @@ typedef struct ljx_light_desc {
     double scale;                  /* envmap radiance scale */
 } ljx_light_desc;
 
-/* One participating medium (src/medium.h:10-22). */
+/* A spectrum-valued volume (src/volume.h:13-31).  A lookup at p returns
+ * scale * value for a constant volume; for a grid, zero outside the box
+ * [p_min, p_max] and otherwise scale times the trilinear interpolation of the
+ * voxels (src/volume.h:40-81), a voxel being channels floats (1: grey, the
+ * same value in R, G and B; 3: RGB) at data[channels * ((z * ry + y) * rx + x)].
+ * The majorant of a grid is scale times the per-channel maximum over its
+ * voxels (at least 0).  The caller owns data; ljx_scene_create copies it. */
+typedef struct ljx_volume_desc {
+    uint32_t type;                 /* LJX_VOLUME_* */
+    int32_t channels;              /* grid: 1 or 3 */
+    int32_t resolution[3];         /* grid: rx, ry, rz >= 1 */
+    int32_t pad;
+    double value[3];               /* constant */
+    double p_min[3];               /* grid box */
+    double p_max[3];
+    double scale;                  /* set_scale (src/volume.h:104-118) */
+    const float *data;             /* grid voxels */
+} ljx_volume_desc;
+
+/* One participating medium (src/medium.h:10-22).  Homogeneous media use
+ * sigma_a and sigma_s; heterogeneous ones sigma_s = density * albedo and
+ * sigma_a = density * (1 - albedo) at each point, with the majorant of the
+ * density volume (src/media/heterogeneous.inl:3-21). */
 typedef struct ljx_medium_desc {
-    uint32_t type;                 /* LJX_MEDIUM_* (homogeneous only here) */
+    uint32_t type;                 /* LJX_MEDIUM_* */
     double sigma_a[3];
     double sigma_s[3];
     uint32_t phase_type;           /* LJX_PHASE_* */
     double g;                      /* Henyey-Greenstein asymmetry */
+    ljx_volume_desc albedo;        /* heterogeneous */
+    ljx_volume_desc density;       /* heterogeneous */
 } ljx_medium_desc;
 
 /* Pinhole camera (src/camera.h:7-22).  fov is the x-axis field of view in

@@ -30,6 +30,7 @@ SHAPE_SPHERE, SHAPE_TRIANGLE_MESH = 0, 1
  MAT_DISNEY_BSDF) = range(9)
 LIGHT_DIFFUSE_AREA, LIGHT_ENVMAP = 0, 1
 MEDIUM_HOMOGENEOUS, MEDIUM_HETEROGENEOUS = 0, 1
+VOLUME_CONSTANT, VOLUME_GRID = 0, 1
 PHASE_ISOTROPIC, PHASE_HENYEY_GREENSTEIN = 0, 1
 FILTER_BOX, FILTER_TENT, FILTER_GAUSSIAN = 0, 1, 2
 (INTEGRATOR_DEPTH, INTEGRATOR_SHADING_NORMAL, INTEGRATOR_MEAN_CURVATURE,
@@ -93,6 +94,20 @@ class LightDesc(C.Structure):
     ]
 
 
+class VolumeDesc(C.Structure):
+    _fields_ = [
+        ("type", C.c_uint32),
+        ("channels", C.c_int32),
+        ("resolution", C.c_int32 * 3),
+        ("pad", C.c_int32),
+        ("value", D3),
+        ("p_min", D3),
+        ("p_max", D3),
+        ("scale", C.c_double),
+        ("data", C.POINTER(C.c_float)),
+    ]
+
+
 class MediumDesc(C.Structure):
     _fields_ = [
         ("type", C.c_uint32),
@@ -100,6 +115,8 @@ class MediumDesc(C.Structure):
         ("sigma_s", D3),
         ("phase_type", C.c_uint32),
         ("g", C.c_double),
+        ("albedo", VolumeDesc),
+        ("density", VolumeDesc),
     ]
 
 

@@ -146,10 +146,26 @@ struct LightRec {
     double to_world[9], to_local[9];  // 3x3 (vectors only)
     double scale;
 };
-struct MediumRec {  // src/medium.h:10-22, homogeneous
+// A spectrum-valued volume (src/volume.h:13-31): a constant, or a grid of
+// FP32 voxels (1 channel: grey, 3: RGB; x fastest) over a box, read with
+// FP64 trilinear interpolation.  max_value is scale times the per-channel
+// maximum over the voxels (at least 0), or scale * value.
+struct VolumeRec {
+    int type;  // 0 constant, 1 grid
+    int channels;
+    int rx, ry, rz;
+    D3 value;
+    D3 p_min, p_max;
+    double scale;
+    D3 max_value;
+    const float *data;
+};
+struct MediumRec {  // src/medium.h:10-22
+    int type;  // 0 homogeneous, 1 heterogeneous
     D3 sigma_a, sigma_s;
     int phase_type;
     double g;
+    VolumeRec albedo, density;
 };
 struct CameraRec {  // src/camera.h:7-22
     double sample_to_cam[16];
@@ -686,6 +702,71 @@ LJX_HD BsdfSample bsdf_sample(const MatRec &m, D3 in, const ShadeGeom &g, double
     case 7: return cos_sample(in, g, u0, u1, 1.0);
     default: return disney_bsdf_sample(m, in, g, u0, u1, w);
     }
+}
+
+// ----------------------------------------------------------------- media
+LJX_HD D3 voxel(const VolumeRec &v, int x, int y, int z) {
+    const float *q = v.data + size_t(v.channels) * ((size_t(z) * v.ry + y) * v.rx + x);
+    if (v.channels == 1) {
+        double a = q[0];
+        return D3{a, a, a};
+    }
+    return D3{double(q[0]), double(q[1]), double(q[2])};
+}
+
+// src/volume.h:40-81: trilinear interpolation, zero outside the box
+LJX_HD D3 vol_lookup(const VolumeRec &v, D3 p) {
+    if (v.type == 0) return v.scale * v.value;
+    D3 pn = (p - v.p_min) / (v.p_max - v.p_min);
+    if (pn.x < 0 || pn.x > 1 || pn.y < 0 || pn.y > 1 || pn.z < 0 || pn.z > 1) return D3{0, 0, 0};
+    pn.x *= double(v.rx - 1);
+    pn.y *= double(v.ry - 1);
+    pn.z *= double(v.rz - 1);
+    const int x0 = min(max(int(pn.x), 0), v.rx - 1), x1 = min(x0 + 1, v.rx - 1);
+    const int y0 = min(max(int(pn.y), 0), v.ry - 1), y1 = min(y0 + 1, v.ry - 1);
+    const int z0 = min(max(int(pn.z), 0), v.rz - 1), z1 = min(z0 + 1, v.rz - 1);
+    const double dx = pn.x - x0, dy = pn.y - y0, dz = pn.z - z0;
+    D3 acc = voxel(v, x0, y0, z0) * ((1 - dx) * (1 - dy) * (1 - dz)) + voxel(v, x1, y0, z0) * (dx * (1 - dy) * (1 - dz)) +
+             voxel(v, x0, y1, z0) * ((1 - dx) * dy * (1 - dz)) + voxel(v, x1, y1, z0) * (dx * dy * (1 - dz)) +
+             voxel(v, x0, y0, z1) * ((1 - dx) * (1 - dy) * dz) + voxel(v, x1, y0, z1) * (dx * (1 - dy) * dz) +
+             voxel(v, x0, y1, z1) * ((1 - dx) * dy * dz) + voxel(v, x1, y1, z1) * (dx * dy * dz);
+    return v.scale * acc;
+}
+
+// src/volume.h:120-144: does the ray's [0, tfar] reach the grid box (pbrt's slab test)
+LJX_HD bool vol_hit(const VolumeRec &v, D3 o, D3 d, double tfar) {
+    if (v.type == 0) return true;
+    double t0 = 0, t1 = tfar;
+    for (int i = 0; i < 3; i++) {
+        double tn = (comp(v.p_min, i) - comp(o, i)) / comp(d, i);
+        double tf = (comp(v.p_max, i) - comp(o, i)) / comp(d, i);
+        if (tn > tf) {
+            double t = tn;
+            tn = tf;
+            tf = t;
+        }
+        t0 = tn > t0 ? tn : t0;
+        t1 = tf < t1 ? tf : t1;
+        if (t0 > t1) return false;
+    }
+    return true;
+}
+
+// src/medium.cpp:27-37, media/homogeneous.inl:1-11, media/heterogeneous.inl:3-21
+LJX_HD D3 medium_majorant(const MediumRec &m, D3 o, D3 d, double tfar) {
+    if (m.type == 0) return m.sigma_a + m.sigma_s;
+    return vol_hit(m.density, o, d, tfar) ? m.density.max_value : D3{0, 0, 0};
+}
+// sigma_s and sigma_a at p
+LJX_HD void medium_sigmas(const MediumRec &m, D3 p, D3 &sig_s, D3 &sig_a) {
+    if (m.type == 0) {
+        sig_s = m.sigma_s;
+        sig_a = m.sigma_a;
+        return;
+    }
+    D3 dens = vol_lookup(m.density, p), alb = vol_lookup(m.albedo, p);
+    sig_s = dens * alb;
+    sig_a = dens * (1.0 - alb);
 }
 
 // ----------------------------------------------------------------- phase

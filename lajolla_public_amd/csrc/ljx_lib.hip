@@ -209,9 +209,20 @@ void validate(const ljx_scene_desc *d) {
     for (int i = 0; i < d->num_materials; i++)
         if (d->materials[i].type >= LJX_MAT_COUNT) fail(LJX_ERR_INVALID_ARGUMENT, "unknown material type");
     for (int i = 0; i < d->num_media; i++) {
-        if (d->media[i].type != LJX_MEDIUM_HOMOGENEOUS)
-            fail(LJX_ERR_UNSUPPORTED, "heterogeneous media are not supported by this build");
-        if (d->media[i].phase_type > 1) fail(LJX_ERR_INVALID_ARGUMENT, "unknown phase function");
+        const ljx_medium_desc &m = d->media[i];
+        if (m.type > LJX_MEDIUM_HETEROGENEOUS) fail(LJX_ERR_INVALID_ARGUMENT, "unknown medium type");
+        if (m.phase_type > 1) fail(LJX_ERR_INVALID_ARGUMENT, "unknown phase function");
+        if (m.type == LJX_MEDIUM_HETEROGENEOUS) {
+            for (const ljx_volume_desc *v : {&m.albedo, &m.density}) {
+                if (v->type > LJX_VOLUME_GRID) fail(LJX_ERR_INVALID_ARGUMENT, "unknown volume type");
+                if (v->type == LJX_VOLUME_GRID &&
+                    (!v->data || (v->channels != 1 && v->channels != 3) || v->resolution[0] < 1 ||
+                     v->resolution[1] < 1 || v->resolution[2] < 1 ||
+                     double(v->resolution[0]) * v->resolution[1] * v->resolution[2] * v->channels > double(1u << 31)))
+                    fail(LJX_ERR_INVALID_ARGUMENT,
+                         "medium " + std::to_string(i) + ": grid volumes need data, 1 or 3 channels and a positive resolution");
+            }
+        }
     }
     for (int i = 0; i < d->num_shapes; i++) {
         const ljx_shape_desc &s = d->shapes[i];
@@ -463,12 +474,45 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
 
     // ---- media
     std::vector<MediumRec> media(d->num_media);
+    auto volume_rec = [&](const ljx_volume_desc &v) {
+        VolumeRec r{};
+        r.type = int(v.type);
+        r.scale = v.scale;
+        r.value = D3{v.value[0], v.value[1], v.value[2]};
+        r.rx = r.ry = r.rz = 1;
+        r.channels = 3;
+        if (v.type == LJX_VOLUME_CONSTANT) {
+            r.max_value = r.scale * r.value;
+            return r;
+        }
+        r.channels = v.channels;
+        r.rx = v.resolution[0], r.ry = v.resolution[1], r.rz = v.resolution[2];
+        r.p_min = D3{v.p_min[0], v.p_min[1], v.p_min[2]};
+        r.p_max = D3{v.p_max[0], v.p_max[1], v.p_max[2]};
+        const size_t n = size_t(r.rx) * r.ry * r.rz;
+        std::vector<float> data(v.data, v.data + n * r.channels);
+        // per-channel maximum from 0, as load_volume keeps it (volume.cpp:70-93)
+        D3 mx{0, 0, 0};
+        for (size_t k = 0; k < n; k++) {
+            const float *q = &data[k * r.channels];
+            D3 x = r.channels == 1 ? D3{q[0], q[0], q[0]} : D3{q[0], q[1], q[2]};
+            mx = D3{std::max(mx.x, x.x), std::max(mx.y, x.y), std::max(mx.z, x.z)};
+        }
+        r.max_value = r.scale * mx;
+        r.data = sc.upload(data);
+        return r;
+    };
     for (int i = 0; i < d->num_media; i++) {
         const ljx_medium_desc &m = d->media[i];
+        media[i].type = int(m.type);
         media[i].sigma_a = D3{m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]};
         media[i].sigma_s = D3{m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]};
         media[i].phase_type = int(m.phase_type);
         media[i].g = m.g;
+        if (m.type == LJX_MEDIUM_HETEROGENEOUS) {
+            media[i].albedo = volume_rec(m.albedo);
+            media[i].density = volume_rec(m.density);
+        }
     }
 
     // ---- camera (src/camera.cpp:7-21)

@@ -4,12 +4,12 @@
 //
 // Covered: <default>, integrators path / volpath / direct and the auxiliary
 // views (depth, shading normal, mean curvature, ray differential, mipmap
-// level), the perspective
-// sensor with fovAxis, film and reconstruction filter, sampler sampleCount,
-// every BSDF of the reference with constant textures, obj / sphere /
-// rectangle shapes, area / point / directional emitters, homogeneous media,
-// and nested transforms.  Image textures, envmap images, .serialized and
-// .ply meshes and heterogeneous media are reported as LJX_ERR_UNSUPPORTED.
+// level), the perspective sensor with fovAxis, film and reconstruction
+// filter, sampler sampleCount, every BSDF of the reference with constant
+// textures, obj / sphere / rectangle shapes, area / point / directional
+// emitters, homogeneous and heterogeneous media (constvolume and gridvolume
+// from Mitsuba .vol files), and nested transforms.  Image textures, envmap
+// images, .serialized and .ply meshes are reported as LJX_ERR_UNSUPPORTED.
 //
 // Numbers are read as FP32 and widened, as the reference's std::stof does
 // (parse_scene.cpp:90-96); relative file names resolve against the scene
@@ -755,32 +755,110 @@ class SceneReader {
         }
     }
 
+    void phase(const XNode &c, ljx_medium_desc &m) {
+        std::string pt = c.get("type");
+        if (pt == "isotropic") m.phase_type = LJX_PHASE_ISOTROPIC;
+        else if (pt == "hg") {
+            m.phase_type = LJX_PHASE_HENYEY_GREENSTEIN;
+            for (const XNode &g : c.kids)
+                if (g.get("name") == "g") m.g = real(g.get("value"), defaults_);
+        } else fail(LJX_ERR_INVALID_ARGUMENT, "unrecognised phase function: " + pt);
+    }
+
+    // load_volume (volume.cpp:6-114): Mitsuba's .vol, version 3, float32
+    // voxels with 1 or 3 channels, x fastest, then the box.
+    void load_vol(const std::string &file, ljx_volume_desc &v) {
+        std::ifstream f(file, std::ios::binary);
+        if (!f) fail(LJX_ERR_INVALID_ARGUMENT, "cannot open volume file " + file);
+        char hdr[4] = {0, 0, 0, 0};
+        f.read(hdr, 4);
+        if (!f || hdr[0] != 'V' || hdr[1] != 'O' || hdr[2] != 'L' || uint8_t(hdr[3]) != 3)
+            fail(LJX_ERR_INVALID_ARGUMENT, "not a version-3 .vol file: " + file);
+        int32_t head[5];
+        f.read(reinterpret_cast<char *>(head), sizeof(head));
+        if (!f) fail(LJX_ERR_INVALID_ARGUMENT, "truncated .vol header: " + file);
+        if (head[0] != 1) fail(LJX_ERR_UNSUPPORTED, "only float32 .vol grids are supported: " + file);
+        const int rx = head[1], ry = head[2], rz = head[3], ch = head[4];
+        if (ch != 1 && ch != 3) fail(LJX_ERR_UNSUPPORTED, ".vol grids have 1 or 3 channels: " + file);
+        if (rx < 1 || ry < 1 || rz < 1 || double(rx) * ry * rz * ch > double(1u << 31))
+            fail(LJX_ERR_INVALID_ARGUMENT, "bad .vol resolution: " + file);
+        float box[6];
+        f.read(reinterpret_cast<char *>(box), sizeof(box));
+        auto buf = std::make_unique<std::vector<float>>(size_t(rx) * ry * rz * ch);
+        f.read(reinterpret_cast<char *>(buf->data()), std::streamsize(buf->size() * sizeof(float)));
+        if (!f) fail(LJX_ERR_INVALID_ARGUMENT, "truncated .vol data: " + file);
+        v.type = LJX_VOLUME_GRID;
+        v.channels = ch;
+        v.resolution[0] = rx, v.resolution[1] = ry, v.resolution[2] = rz;
+        for (int k = 0; k < 3; k++) v.p_min[k] = box[k], v.p_max[k] = box[3 + k];
+        v.scale = 1;
+        v.data = buf->data();
+        P.fbufs.push_back(std::move(buf));
+    }
+
+    // parse_volume_spectrum (parse_scene.cpp:643-672)
+    ljx_volume_desc volume(const XNode &n) {
+        ljx_volume_desc v{};
+        v.scale = 1;
+        std::string t = n.get("type");
+        if (t == "constvolume") {
+            v.type = LJX_VOLUME_CONSTANT;
+            for (const XNode &c : n.kids)
+                if (c.get("name") == "value") {
+                    V3 x = color(c, defaults_);
+                    v.value[0] = x.x, v.value[1] = x.y, v.value[2] = x.z;
+                }
+        } else if (t == "gridvolume") {
+            std::string file;
+            for (const XNode &c : n.kids)
+                if (c.get("name") == "filename") file = resolve(c.get("value"), defaults_);
+            if (file.empty()) fail(LJX_ERR_INVALID_ARGUMENT, "gridvolume without a filename");
+            load_vol(path(file), v);
+        } else {
+            fail(LJX_ERR_INVALID_ARGUMENT, "unknown volume type: " + t);
+        }
+        return v;
+    }
+
+    // parse_medium (parse_scene.cpp:692-746)
     std::pair<std::string, ljx_medium_desc> medium(const XNode &n) {
         ljx_medium_desc m{};
         std::string t = n.get("type");
-        if (t != "homogeneous") fail(LJX_ERR_UNSUPPORTED, "only homogeneous media are supported: " + t);
-        m.type = LJX_MEDIUM_HOMOGENEOUS;
-        V3 sa{0.5, 0.5, 0.5}, ss{0.5, 0.5, 0.5};
-        double scale = 1;
         m.phase_type = LJX_PHASE_ISOTROPIC;
-        for (const XNode &c : n.kids) {
-            std::string nm = c.get("name");
-            if (nm == "sigmaA" || nm == "sigma_a") sa = color(c, defaults_);
-            else if (nm == "sigmaS" || nm == "sigma_s") ss = color(c, defaults_);
-            else if (nm == "scale") scale = real(c.get("value"), defaults_);
-            else if (c.name == "phase") {
-                std::string pt = c.get("type");
-                if (pt == "isotropic") m.phase_type = LJX_PHASE_ISOTROPIC;
-                else if (pt == "hg") {
-                    m.phase_type = LJX_PHASE_HENYEY_GREENSTEIN;
-                    for (const XNode &g : c.kids)
-                        if (g.get("name") == "g") m.g = real(g.get("value"), defaults_);
-                } else fail(LJX_ERR_INVALID_ARGUMENT, "unrecognised phase function: " + pt);
+        double scale = 1;
+        if (t == "homogeneous") {
+            m.type = LJX_MEDIUM_HOMOGENEOUS;
+            V3 sa{0.5, 0.5, 0.5}, ss{0.5, 0.5, 0.5};
+            for (const XNode &c : n.kids) {
+                std::string nm = c.get("name");
+                if (nm == "sigmaA" || nm == "sigma_a") sa = color(c, defaults_);
+                else if (nm == "sigmaS" || nm == "sigma_s") ss = color(c, defaults_);
+                else if (nm == "scale") scale = real(c.get("value"), defaults_);
+                else if (c.name == "phase") phase(c, m);
             }
+            // the parser's scale multiplies both coefficients (parse_scene.cpp:704-722)
+            m.sigma_a[0] = sa.x * scale, m.sigma_a[1] = sa.y * scale, m.sigma_a[2] = sa.z * scale;
+            m.sigma_s[0] = ss.x * scale, m.sigma_s[1] = ss.y * scale, m.sigma_s[2] = ss.z * scale;
+        } else if (t == "heterogeneous") {
+            m.type = LJX_MEDIUM_HETEROGENEOUS;
+            // both volumes default to the constant 1 (parse_scene.cpp:724-725)
+            for (ljx_volume_desc *v : {&m.albedo, &m.density}) {
+                v->type = LJX_VOLUME_CONSTANT;
+                v->value[0] = v->value[1] = v->value[2] = 1;
+                v->scale = 1;
+            }
+            for (const XNode &c : n.kids) {
+                std::string nm = c.get("name");
+                if (nm == "albedo") m.albedo = volume(c);
+                else if (nm == "density") m.density = volume(c);
+                else if (nm == "scale") scale = real(c.get("value"), defaults_);
+                else if (c.name == "phase") phase(c, m);
+            }
+            // the scale applies to the density only (parse_scene.cpp:739-740)
+            m.density.scale = scale;
+        } else {
+            fail(LJX_ERR_INVALID_ARGUMENT, "unknown medium type: " + t);
         }
-        // the parser's scale multiplies both coefficients (parse_scene.cpp:704-722)
-        m.sigma_a[0] = sa.x * scale, m.sigma_a[1] = sa.y * scale, m.sigma_a[2] = sa.z * scale;
-        m.sigma_s[0] = ss.x * scale, m.sigma_s[1] = ss.y * scale, m.sigma_s[2] = ss.z * scale;
         return {n.get("id"), m};
     }
 

@@ -1,4 +1,5 @@
-// Volumetric path tracing on the wavefront loop (homogeneous media).
+// Volumetric path tracing on the wavefront loop (homogeneous and
+// heterogeneous media).
 //
 // lajolla's vol_path_tracing is a homework stub returning zero
 // (src/vol_path_tracing.h:59-64); this follows the handout's final
@@ -33,7 +34,17 @@ __global__ void k_reset_stats(unsigned long long *stat) {
     if (threadIdx.x < S_NUM) stat[threadIdx.x] = 0;
 }
 
-__device__ __forceinline__ D3 sigma_t_of(const MediumRec &m) { return m.sigma_a + m.sigma_s; }
+// sigma_t = sigma_s + sigma_a at p
+__device__ __forceinline__ D3 sigma_t_at(const MediumRec &m, D3 p) {
+    D3 s, a;
+    medium_sigmas(m, p, s, a);
+    return s + a;
+}
+__device__ __forceinline__ D3 sigma_s_at(const MediumRec &m, D3 p) {
+    D3 s, a;
+    medium_sigmas(m, p, s, a);
+    return s;
+}
 
 // update_medium (homework2.tex:402-409)
 __device__ __forceinline__ int update_medium(D3 dir, const Vertex &v, int medium) {
@@ -76,7 +87,7 @@ __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &
     D3 p_scatter = org;
     if (medium >= 0) {
         const MediumRec &m = S.media[medium];
-        D3 maj = sigma_t_of(m);
+        D3 maj = medium_majorant(m, org, dir, __builtin_huge_val());
         double max_maj = maxc(maj);
         double u = pcg_double(st, inc);
         int ch = min(max(int(u * 3), 0), 2);
@@ -91,7 +102,7 @@ __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &
             double dt = t_hit - accum_t;
             accum_t = fmin(accum_t + t, t_hit);
             if (t < dt) {
-                D3 sig_t = sigma_t_of(m);
+                D3 sig_t = sigma_t_at(m, org + accum_t * dir);
                 D3 real_prob = sig_t / maj;
                 D3 e = vexp(-1.0 * maj * t);
                 if (pcg_double(st, inc) < comp(real_prob, ch)) {
@@ -167,7 +178,7 @@ __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &
                     double pv = phase_value(m, dir_view, dir_light);
                     f = D3{pv, pv, pv};
                     pdf_f = pv;
-                    pre = pre * m.sigma_s;
+                    pre = pre * sigma_s_at(m, p_scatter);
                 } else {
                     const MatRec &mat = S.materials[v.material_id];
                     f = bsdf_eval<-1>(mat, dir_view, dir_light, v.g);
@@ -193,7 +204,7 @@ __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &
             if (pv <= 0) {
                 alive = false;
             } else {
-                tp = tp * (D3{pv, pv, pv} / pv) * m.sigma_s;
+                tp = tp * (D3{pv, pv, pv} / pv) * sigma_s_at(m, p_scatter);
                 dir_pdf = pv;
                 nee_p_cache = p_scatter;
                 multi_trans_pdf = D3{1, 1, 1};
@@ -316,7 +327,7 @@ __global__ __launch_bounds__(256) void k_vol_shadow(DeviceScene S, PathBufs P) {
             }
             if (medium >= 0) {
                 const MediumRec &m = S.media[medium];
-                D3 maj = sigma_t_of(m);
+                D3 maj = medium_majorant(m, p0, dir_light, (1 - S.eps) * dl);
                 double max_maj = maxc(maj);
                 double u = pcg_double(nst, ninc);
                 int ch = min(max(int(u * 3), 0), 2);
@@ -329,7 +340,7 @@ __global__ __launch_bounds__(256) void k_vol_shadow(DeviceScene S, PathBufs P) {
                     double dt = next_t - accum_t;
                     accum_t = fmin(accum_t + t, next_t);
                     if (t < dt) {
-                        D3 sig_t = sigma_t_of(m);
+                        D3 sig_t = sigma_t_at(m, p0 + accum_t * dir_light);
                         D3 sig_n = maj - sig_t;
                         D3 e = vexp(-1.0 * maj * t);
                         T = T * e * sig_n / max_maj;

@@ -48,6 +48,36 @@ def look_at(pos, look, up):
     return m
 
 
+def constant_volume(value, scale=1.0):
+    """ConstantVolume<Spectrum> (src/volume.h:13-16)."""
+    return dict(type=capi.VOLUME_CONSTANT, value=tuple(float(v) for v in value), scale=float(scale))
+
+
+def grid_volume(data, p_min, p_max, scale=1.0):
+    """GridVolume<Spectrum> (src/volume.h:18-26). data: (rz, ry, rx) or
+    (rz, ry, rx, 1|3) floats, x fastest in memory."""
+    a = np.ascontiguousarray(data, np.float32)
+    if a.ndim == 3:
+        a = a[..., None]
+    assert a.ndim == 4 and a.shape[3] in (1, 3)
+    return dict(type=capi.VOLUME_GRID, data=a, p_min=tuple(map(float, p_min)), p_max=tuple(map(float, p_max)),
+                scale=float(scale))
+
+
+def write_vol(path, data, p_min, p_max):
+    """A Mitsuba .vol grid (version 3, float32; the layout read by
+    src/volume.cpp:6-114): data (rz, ry, rx) or (rz, ry, rx, 1|3)."""
+    a = np.ascontiguousarray(data, np.float32)
+    if a.ndim == 3:
+        a = a[..., None]
+    rz, ry, rx, ch = a.shape
+    with open(path, "wb") as f:
+        f.write(b"VOL" + bytes([3]))
+        f.write(np.array([1, rx, ry, rz, ch], np.int32).tobytes())
+        f.write(np.array(list(p_min) + list(p_max), np.float32).tobytes())
+        f.write(a.tobytes())
+
+
 def camera_desc(cam):
     """ljx_camera_desc from a camera dict (cam_to_world, fov, width, height,
     filter_type, filter_param, medium_id)."""
@@ -124,7 +154,13 @@ class SceneDesc:
 
     def add_medium(self, sigma_a, sigma_s, phase_type=capi.PHASE_ISOTROPIC, g=0.0):
         self.media.append(dict(type=capi.MEDIUM_HOMOGENEOUS, sigma_a=tuple(sigma_a),
-                               sigma_s=tuple(sigma_s), phase_type=phase_type, g=g))
+                               sigma_s=tuple(sigma_s), phase_type=phase_type, g=g, albedo=None, density=None))
+        return len(self.media) - 1
+
+    def add_heterogeneous_medium(self, albedo, density, phase_type=capi.PHASE_ISOTROPIC, g=0.0):
+        """albedo and density are volumes from constant_volume / grid_volume."""
+        self.media.append(dict(type=capi.MEDIUM_HETEROGENEOUS, sigma_a=(0, 0, 0), sigma_s=(0, 0, 0),
+                               phase_type=phase_type, g=g, albedo=albedo, density=density))
         return len(self.media) - 1
 
     @property
@@ -189,6 +225,22 @@ class SceneDesc:
             o.sigma_s = capi.D3(*m["sigma_s"])
             o.phase_type = m["phase_type"]
             o.g = m["g"]
+            for name in ("albedo", "density"):
+                v = m.get(name)
+                if v is None:
+                    continue
+                ov = getattr(o, name)
+                ov.type = v["type"]
+                ov.scale = v["scale"]
+                if v["type"] == capi.VOLUME_CONSTANT:
+                    ov.value = capi.D3(*v["value"])
+                else:
+                    data = v["data"]
+                    ov.channels = data.shape[3]
+                    ov.resolution[0], ov.resolution[1], ov.resolution[2] = data.shape[2], data.shape[1], data.shape[0]
+                    ov.p_min = capi.D3(*v["p_min"])
+                    ov.p_max = capi.D3(*v["p_max"])
+                    ov.data = ptr(data, C.c_float)
         keep += [shapes, mats, lights, media]
         d.shapes = C.cast(shapes, C.POINTER(capi.ShapeDesc))
         d.num_shapes = len(self.shapes)
@@ -591,4 +643,63 @@ def nested_media(seed=11, g=0.7, width=1024, height=1024, spp=128, max_depth=16,
     s.options.update(integrator=capi.INTEGRATOR_VOLPATH, samples_per_pixel=spp, max_depth=max_depth,
                      rr_depth=5)
     s.meta = dict(seed=seed, g=g, sphere_res=sphere_res)
+    return s
+
+
+def smoke_density(res, seed=3, octaves=4):
+    """A seeded, wispy smoke-like density on a res^3 grid (values in [0, ~1]):
+    octaves of trilinearly upsampled value noise, thresholded, with a
+    spherical falloff so the grid's border holds nothing."""
+    rng = np.random.default_rng(seed)
+    n = res
+    g = np.linspace(0, 1, n)
+    acc = np.zeros((n, n, n))
+    amp, base = 1.0, 4
+    for _ in range(octaves):
+        coarse = rng.uniform(-1, 1, (base + 1,) * 3)
+        idx = g * base
+        i0 = np.minimum(idx.astype(int), base - 1)
+        f = idx - i0
+        # separable trilinear upsampling
+        a = coarse[i0] * (1 - f)[:, None, None] + coarse[i0 + 1] * f[:, None, None]
+        a = a[:, i0] * (1 - f)[None, :, None] + a[:, i0 + 1] * f[None, :, None]
+        a = a[:, :, i0] * (1 - f)[None, None, :] + a[:, :, i0 + 1] * f[None, None, :]
+        acc += amp * a
+        amp *= 0.5
+        base *= 2
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    r = np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2)
+    falloff = np.clip(1 - (r / 0.5) ** 2, 0, 1)
+    return (np.clip(acc + 0.2, 0, None) * falloff).astype(np.float32)
+
+
+def hetvol_smoke(seed=3, width=768, height=576, spp=64, max_depth=16, res=128, scale=40.0, colored=False):
+    """Heterogeneous-medium scene in the manner of the reference's
+    scenes/volpath_test/hetvol.xml: a res^3 smoke density grid (times scale)
+    with a constant albedo of 0.9 (or a seeded RGB albedo grid when colored),
+    inside an index-matched unit box above a diffuse plane, lit by a sphere
+    emitter of radiance 16; Henyey-Greenstein g = 0."""
+    s = SceneDesc()
+    s.name = "hetvol_smoke"
+    lo, hi = (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)
+    dens = grid_volume(smoke_density(res, seed), lo, hi, scale)
+    if colored:
+        rng = np.random.default_rng(seed + 1)
+        alb = grid_volume(rng.uniform(0.6, 0.98, (8, 8, 8, 3)), lo, hi)
+    else:
+        alb = constant_volume((0.9, 0.9, 0.9))
+    med = s.add_heterogeneous_medium(alb, dens, capi.PHASE_HENYEY_GREENSTEIN, 0.0)
+    P, I, _ = box((0, 0, 0), (1.0, 1.0, 1.0), 0.0)
+    s.add_mesh(P, I, -1, interior_medium_id=med, exterior_medium_id=-1)
+    m_floor = s.add_material(capi.MAT_LAMBERTIAN, (0.2, 0.2, 0.3))
+    Pf, If, Nf = quad((-4, -0.52, -4), (-4, -0.52, 4), (4, -0.52, 4), (4, -0.52, -4))
+    s.add_mesh(Pf, If, m_floor, normals=Nf)
+    m_black = s.add_material(capi.MAT_LAMBERTIAN, (0, 0, 0))
+    lid = s.add_sphere((-0.8, 2.0, -1.0), 0.6, m_black)
+    s.add_area_light(lid, (16.0, 16.0, 16.0))
+    s.camera.update(cam_to_world=look_at((0.4, 0.5, 2.2), (0, 0, 0), (0, 1, 0)), fov=45.0,
+                    width=width, height=height, filter_type=capi.FILTER_BOX, filter_param=1.0)
+    s.options.update(integrator=capi.INTEGRATOR_VOLPATH, samples_per_pixel=spp, max_depth=max_depth,
+                     rr_depth=5)
+    s.meta = dict(seed=seed, res=res, scale=scale, colored=colored)
     return s

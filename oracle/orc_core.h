@@ -68,6 +68,7 @@ inline V3 normalize(V3 a) {
     return a / l;
 }
 inline Real max3(V3 a) { return std::max(std::max(a.x, a.y), a.z); }  // vector.h:265
+inline V3 vmax(V3 a, V3 b) { return {std::max(a.x, b.x), std::max(a.y, b.y), std::max(a.z, b.z)}; }
 inline Real avg(V3 a) { return (a.x + a.y + a.z) / 3; }               // spectrum.h:36
 inline V3 vexp(V3 a) { return {std::exp(a.x), std::exp(a.y), std::exp(a.z)}; }
 inline V3 vsqrt0(V3 a) {  // spectrum.h:21-25

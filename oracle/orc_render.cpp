@@ -46,9 +46,29 @@ struct Light {  // light.h:12-30
     Real scale = 1;
     TableDist2D dist;
 };
-struct Medium {  // medium.h:10-22 (homogeneous)
+// volume.h:13-31: a constant or a grid of spectra.  Grey grids (one float
+// per voxel) hold the same value in every channel, as load_volume's
+// fromRGB(v, v, v) (volume.cpp:80-88).
+struct Volume {
+    int type = LJX_VOLUME_CONSTANT;
+    V3 value;
+    int res[3] = {1, 1, 1};
+    int channels = 3;
+    V3 p_min, p_max;
+    Real scale = 1;
+    std::vector<float> data;
+    V3 max_data;  // per-channel maximum over the voxels, from 0 (volume.cpp:75-93)
+    V3 voxel(int x, int y, int z) const {
+        size_t i = size_t(channels) * ((size_t(z) * res[1] + y) * res[0] + x);
+        return channels == 1 ? V3{Real(data[i]), Real(data[i]), Real(data[i])}
+                             : V3{Real(data[i]), Real(data[i + 1]), Real(data[i + 2])};
+    }
+};
+struct Medium {  // medium.h:10-22
+    int type = LJX_MEDIUM_HOMOGENEOUS;
     V3 sigma_a, sigma_s;
     Phase phase;
+    Volume albedo, density;
 };
 struct PathVertex {  // intersection.h:15-35
     V3 position, geometric_normal;
@@ -708,6 +728,56 @@ static V3 path_tracing(const Scene &s, int x, int y, Pcg &rng, Counters &cnt) {
 // surfaces (:770-812, :460-507), update_medium (:402-409), the MIS caches
 // dir_pdf / nee_p_cache / multi_trans_pdf / never_scatter (:505-557), and
 // Russian roulette from rr_depth (:331-400).
+// volume.h:40-81: trilinear interpolation, zero outside the box.
+static V3 lookup(const Volume &v, const V3 &p) {
+    if (v.type == LJX_VOLUME_CONSTANT) return v.scale * v.value;
+    V3 pn = (p - v.p_min) / (v.p_max - v.p_min);
+    if (pn.x < 0 || pn.x > 1 || pn.y < 0 || pn.y > 1 || pn.z < 0 || pn.z > 1) return {};
+    pn.x *= Real(v.res[0] - 1);
+    pn.y *= Real(v.res[1] - 1);
+    pn.z *= Real(v.res[2] - 1);
+    int x0 = std::clamp(int(pn.x), 0, v.res[0] - 1), x1 = std::clamp(x0 + 1, 0, v.res[0] - 1);
+    int y0 = std::clamp(int(pn.y), 0, v.res[1] - 1), y1 = std::clamp(y0 + 1, 0, v.res[1] - 1);
+    int z0 = std::clamp(int(pn.z), 0, v.res[2] - 1), z1 = std::clamp(z0 + 1, 0, v.res[2] - 1);
+    Real dx = pn.x - x0, dy = pn.y - y0, dz = pn.z - z0;
+    V3 acc = v.voxel(x0, y0, z0) * ((1 - dx) * (1 - dy) * (1 - dz)) + v.voxel(x1, y0, z0) * (dx * (1 - dy) * (1 - dz)) +
+             v.voxel(x0, y1, z0) * ((1 - dx) * dy * (1 - dz)) + v.voxel(x1, y1, z0) * (dx * dy * (1 - dz)) +
+             v.voxel(x0, y0, z1) * ((1 - dx) * (1 - dy) * dz) + v.voxel(x1, y0, z1) * (dx * (1 - dy) * dz) +
+             v.voxel(x0, y1, z1) * ((1 - dx) * dy * dz) + v.voxel(x1, y1, z1) * (dx * dy * dz);
+    return v.scale * acc;
+}
+// volume.h:86-101
+static V3 max_value(const Volume &v) {
+    return v.type == LJX_VOLUME_CONSTANT ? v.scale * v.value : v.scale * v.max_data;
+}
+// volume.h:120-144: does the ray's [0, tfar] reach the grid box (pbrt's slab test)
+static bool volume_hit(const Volume &v, const Ray &ray) {
+    if (v.type == LJX_VOLUME_CONSTANT) return true;
+    Real t0 = 0, t1 = ray.tfar;
+    for (int i = 0; i < 3; i++) {
+        Real tn = (v.p_min[i] - ray.org[i]) / ray.dir[i];
+        Real tf = (v.p_max[i] - ray.org[i]) / ray.dir[i];
+        if (tn > tf) std::swap(tn, tf);
+        t0 = tn > t0 ? tn : t0;
+        t1 = tf < t1 ? tf : t1;
+        if (t0 > t1) return false;
+    }
+    return true;
+}
+// medium.cpp:27-37 with homogeneous.inl:1-11 and heterogeneous.inl:3-21
+static V3 get_majorant(const Medium &m, const Ray &ray) {
+    if (m.type == LJX_MEDIUM_HOMOGENEOUS) return m.sigma_a + m.sigma_s;
+    return volume_hit(m.density, ray) ? max_value(m.density) : V3{};
+}
+static V3 get_sigma_s(const Medium &m, const V3 &p) {
+    if (m.type == LJX_MEDIUM_HOMOGENEOUS) return m.sigma_s;
+    return lookup(m.density, p) * lookup(m.albedo, p);
+}
+static V3 get_sigma_a(const Medium &m, const V3 &p) {
+    if (m.type == LJX_MEDIUM_HOMOGENEOUS) return m.sigma_a;
+    return lookup(m.density, p) * (1 - lookup(m.albedo, p));
+}
+
 static int update_medium(V3 dir, const PathVertex &v, int medium) {
     if (v.interior_medium_id != v.exterior_medium_id) {
         if (dot(dir, v.geometric_normal) > 0) return v.exterior_medium_id;
@@ -739,7 +809,7 @@ static V3 vol_nee(const Scene &s, V3 p, V3 dir_view, int medium_id, int bounces,
         if (hit) next_t = distance(p0, iv.position);
         if (shadow_medium >= 0) {
             const Medium &m = s.media[shadow_medium];
-            V3 maj = m.sigma_a + m.sigma_s;
+            V3 maj = get_majorant(m, sr);
             Real max_maj = max3(maj);
             Real u = pcg_double(rng);
             int ch = std::clamp(int(u * 3), 0, 2);
@@ -752,7 +822,8 @@ static V3 vol_nee(const Scene &s, V3 p, V3 dir_view, int medium_id, int bounces,
                 Real dt = next_t - accum_t;
                 accum_t = std::min(accum_t + t, next_t);
                 if (t < dt) {
-                    V3 sigma_t = m.sigma_a + m.sigma_s;
+                    V3 pt = p0 + accum_t * dir_light;
+                    V3 sigma_t = get_sigma_s(m, pt) + get_sigma_a(m, pt);
                     V3 sigma_n = maj - sigma_t;
                     V3 e = vexp(-1.0 * maj * t);
                     T_light *= e * sigma_n / max_maj;
@@ -815,7 +886,7 @@ static V3 vol_path_tracing(const Scene &s, int x, int y, Pcg &rng, bool nee_stre
         bool hit = intersect(s, ray, v, &cnt.ext);
         if (medium >= 0) {
             const Medium &m = s.media[medium];
-            V3 maj = m.sigma_a + m.sigma_s;
+            V3 maj = get_majorant(m, ray);
             Real max_maj = max3(maj);
             Real u = pcg_double(rng);
             int ch = std::clamp(int(u * 3), 0, 2);
@@ -830,7 +901,8 @@ static V3 vol_path_tracing(const Scene &s, int x, int y, Pcg &rng, bool nee_stre
                 Real dt = t_hit - accum_t;
                 accum_t = std::min(accum_t + t, t_hit);
                 if (t < dt) {
-                    V3 sigma_t = m.sigma_a + m.sigma_s;
+                    V3 pt = ray.org + accum_t * ray.dir;
+                    V3 sigma_t = get_sigma_s(m, pt) + get_sigma_a(m, pt);
                     V3 real_prob = sigma_t / maj;
                     V3 e = vexp(-1.0 * maj * t);
                     if (pcg_double(rng) < real_prob[ch]) {
@@ -888,15 +960,16 @@ static V3 vol_path_tracing(const Scene &s, int x, int y, Pcg &rng, bool nee_stre
         if (scatter) {
             const Medium &m = s.media[medium];
             V3 p = ray.org;
+            V3 sig_s = get_sigma_s(m, p);
             V3 C = vol_nee(s, p, dir_view, medium, bounces, *nr, nullptr, nullptr, &m.phase, cnt);
-            radiance += throughput * m.sigma_s * C;
+            radiance += throughput * sig_s * C;
             V2 r{pcg_double(rng), pcg_double(rng)};
             std::optional<V3> nd = phase_sample(m.phase, dir_view, r);
             if (!nd) break;
             V3 f = phase_eval(m.phase, dir_view, *nd);
             Real pdf = phase_pdf(m.phase, dir_view, *nd);
             if (pdf <= 0) break;
-            throughput *= f / pdf * m.sigma_s;
+            throughput *= f / pdf * sig_s;
             dir_pdf = pdf;
             nee_p_cache = p;
             multi_trans_pdf = V3{1, 1, 1};
@@ -976,9 +1049,40 @@ static int make_scene(const ljx_scene_desc *d, std::unique_ptr<Scene> &out) {
     for (int i = 0; i < d->num_media; i++) {
         const ljx_medium_desc &m = d->media[i];
         Medium q;
+        q.type = int(m.type);
         q.sigma_a = v3(m.sigma_a);
         q.sigma_s = v3(m.sigma_s);
         q.phase = Phase{int(m.phase_type), m.g};
+        if (q.type == LJX_MEDIUM_HETEROGENEOUS) {
+            for (int k = 0; k < 2; k++) {
+                const ljx_volume_desc &vd = k ? m.density : m.albedo;
+                Volume &vol = k ? q.density : q.albedo;
+                vol.type = int(vd.type);
+                vol.scale = vd.scale;
+                vol.value = v3(vd.value);
+                if (vol.type == LJX_VOLUME_GRID) {
+                    if (!vd.data || (vd.channels != 1 && vd.channels != 3) || vd.resolution[0] < 1 ||
+                        vd.resolution[1] < 1 || vd.resolution[2] < 1) {
+                        g_err = "grid volume: data, channels (1 or 3) and a positive resolution are required";
+                        return LJX_ERR_INVALID_ARGUMENT;
+                    }
+                    vol.channels = vd.channels;
+                    for (int a = 0; a < 3; a++) vol.res[a] = vd.resolution[a];
+                    vol.p_min = v3(vd.p_min);
+                    vol.p_max = v3(vd.p_max);
+                    size_t n = size_t(vd.channels) * vd.resolution[0] * vd.resolution[1] * vd.resolution[2];
+                    vol.data.assign(vd.data, vd.data + n);
+                    V3 mx{};
+                    for (int z = 0; z < vol.res[2]; z++)
+                        for (int y = 0; y < vol.res[1]; y++)
+                            for (int x = 0; x < vol.res[0]; x++) mx = vmax(mx, vol.voxel(x, y, z));
+                    vol.max_data = mx;
+                }
+            }
+        } else if (q.type != LJX_MEDIUM_HOMOGENEOUS) {
+            g_err = "unknown medium type";
+            return LJX_ERR_INVALID_ARGUMENT;
+        }
         s->media.push_back(q);
     }
     Real lo[3] = {inf(), inf(), inf()}, hi[3] = {-inf(), -inf(), -inf()};
