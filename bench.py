@@ -28,9 +28,11 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp", type=int, default=0, help="0: the workload's own spp")
-    ap.add_argument("--workload", choices=["terrain_spheres", "nested_media"], default="terrain_spheres",
+    ap.add_argument("--workload", choices=["terrain_spheres", "nested_media", "hetvol_smoke"],
+                    default="terrain_spheres",
                     help="terrain_spheres (default): the headline 1080p large-mesh workload; nested_media: the "
-                         "1024x1024, 128 spp, depth 16 volumetric workload")
+                         "1024x1024, 128 spp, depth 16 volumetric workload; hetvol_smoke: a 128^3 smoke grid "
+                         "(heterogeneous medium), 768x576, 256 spp, unbounded depth with RR")
     ap.add_argument("--cpu-spp", type=int, default=0, help="spp of the bounded CPU baseline sample (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-paths", type=int, default=0)
@@ -74,6 +76,12 @@ def main():
         workload = ("terrain_spheres: fBm heightfield 1000x500 + 64 icospheres (subdiv 5) + 1024 emissive "
                     "triangles = 2,308,746 triangles, constant environment, 1920x1080, 64 spp, max depth 8")
         metric = "Msamples/s (1080p, depth 8)"
+    elif args.workload == "hetvol_smoke":
+        desc = scenes.hetvol_smoke(seed=3, spp=args.spp or 256, max_depth=-1, res=128, scale=40.0)
+        workload = ("hetvol_smoke: 128^3 smoke density grid x 40 (heterogeneous medium, albedo 0.9, HG g=0) in an "
+                    "index-matched box over a diffuse plane, sphere emitter, 768x576, 256 spp, unbounded depth, RR "
+                    "from depth 5, volpath (delta and ratio tracking)")
+        metric = "Msamples/s (768x576, volpath, heterogeneous)"
     else:
         desc = scenes.nested_media(seed=11, g=0.7, spp=args.spp or 128)
         workload = ("nested_media: 100,352-triangle index-matched sphere (homogeneous medium, HG g=0.7) inside a "
@@ -438,7 +446,8 @@ def cpu_baseline(desc, cpu_spp):
         "unit": "Msamples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"every 16x16 tile of the 1920x1080 image at {spp} spp ({cnt['paths']} camera paths, "
+        "sample": f"every 16x16 tile of the {desc.camera['width']}x{desc.camera['height']} image at {spp} spp "
+                  f"({cnt['paths']} camera paths, "
                   f"{dt:.1f} s; BVH build {build_s:.1f} s not timed); lajolla-algorithm CPU baseline "
                   f"(project BVH, not Embree)",
         "grays_per_s": round((cnt["ext_rays"] + cnt["shadow_rays"]) / dt / 1e9, 5),
