@@ -44,7 +44,7 @@ def build(verbose=True):
         run([HIPCC, *COMMON, *DEVICE, "-I" + CSRC, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = OUT + ".tmp"
-    run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-pthread", "-o", tmp, *objs])
+    run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-pthread", "-o", tmp, *objs, "-lz"])
     os.replace(tmp, OUT)
     # the command-line renderer, a host program over the C++ API
     cli = os.path.join(HERE, "lajolla_amd")

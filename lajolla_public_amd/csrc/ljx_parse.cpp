@@ -1,15 +1,16 @@
 // Scene files: the Mitsuba-style XML subset lajolla's parser reads
-// (src/parsers/parse_scene.cpp) and Wavefront OBJ meshes
-// (src/parsers/parse_obj.cpp), turned into an ljx_scene_desc.
+// (src/parsers/parse_scene.cpp) with Wavefront OBJ (src/parsers/parse_obj.cpp),
+// Mitsuba .serialized (load_serialized.cpp) and PLY (parse_ply.cpp) meshes,
+// turned into an ljx_scene_desc.
 //
 // Covered: <default>, integrators path / volpath / direct and the auxiliary
 // views (depth, shading normal, mean curvature, ray differential, mipmap
 // level), the perspective sensor with fovAxis, film and reconstruction
 // filter, sampler sampleCount, every BSDF of the reference with constant
-// textures, obj / sphere / rectangle shapes, area / point / directional
-// emitters, homogeneous and heterogeneous media (constvolume and gridvolume
-// from Mitsuba .vol files), and nested transforms.  Image textures, envmap
-// images, .serialized and .ply meshes are reported as LJX_ERR_UNSUPPORTED.
+// textures, obj / serialized / ply / sphere / rectangle shapes, area / point /
+// directional emitters, homogeneous and heterogeneous media (constvolume and
+// gridvolume from Mitsuba .vol files), and nested transforms.  Image textures
+// and envmap images are reported as LJX_ERR_UNSUPPORTED.
 //
 // Numbers are read as FP32 and widened, as the reference's std::stof does
 // (parse_scene.cpp:90-96); relative file names resolve against the scene
@@ -17,15 +18,19 @@
 #include "../../include/lajolla_amd.h"
 #include "ljx_host.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <sstream>
 #include <string>
 #include <vector>
+
+#include <zlib.h>
 
 namespace ljx {
 namespace {
@@ -218,9 +223,10 @@ V3 operator*(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
 double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 double length(V3 a) { return std::sqrt(dot(a, a)); }
+// vector.h:249-257
 V3 normalize(V3 a) {
     double l = length(a);
-    return l > 0 ? a * (1 / l) : V3{};
+    return l > 0 ? V3{a.x / l, a.y / l, a.z / l} : V3{};
 }
 V3 xform_point(const M4 &m, V3 p) {
     double x = m(0, 0) * p.x + m(0, 1) * p.y + m(0, 2) * p.z + m(0, 3);
@@ -476,8 +482,11 @@ struct Mesh {
 // (src/parsers/shape_utils.h:15-50).
 std::vector<V3> vertex_normals(const Mesh &m) {
     std::vector<V3> n(m.pos.size());
+    // The corner weight is the reference's unit_angle (shape_utils.h:8-13),
+    // including its (pi - 2) * asin(...) for obtuse corners, where Max's
+    // angle is pi - 2 asin(...): normals computed here are the reference's.
     auto unit_angle = [](V3 u, V3 v) {
-        return dot(u, v) < 0 ? (kPiD - 2 * std::asin(0.5 * length(v + u))) : 2 * std::asin(0.5 * length(v - u));
+        return dot(u, v) < 0 ? (kPiD - 2) * std::asin(0.5 * length(v + u)) : 2 * std::asin(0.5 * length(v - u));
     };
     for (size_t f = 0; f + 2 < m.idx.size(); f += 3) {
         V3 fn;
@@ -488,12 +497,15 @@ std::vector<V3> vertex_normals(const Mesh &m) {
                 fn = cross(s1, s2);
                 double l = length(fn);
                 if (l == 0) break;
-                fn = fn * (1 / l);
+                fn = {fn.x / l, fn.y / l, fn.z / l};
             }
             n[m.idx[f + i]] = n[m.idx[f + i]] + fn * unit_angle(normalize(s1), normalize(s2));
         }
     }
-    for (auto &v : n) v = normalize(v);
+    for (auto &v : n) {
+        double l = length(v);
+        v = l != 0 ? V3{v.x / l, v.y / l, v.z / l} : V3{0, 0, 0};
+    }
     return n;
 }
 
@@ -564,6 +576,288 @@ Mesh load_obj(const std::string &file, const M4 &to_world) {
     }
     if (!m.uv.empty() && m.uv.size() != m.pos.size()) m.uv.clear();
     if (!m.nrm.empty() && m.nrm.size() != m.pos.size()) m.nrm.clear();
+    return m;
+}
+
+// ------------------------------------------------------- .serialized
+// Mitsuba's .serialized meshes (load_serialized.cpp:102-256): a u16 magic and
+// a u16 version (3 or 4).  Shape k > 0 starts at an offset from the table at
+// the end of the file (version 4: u64 offsets, version 3: u32 offsets, then
+// a u32 count), past its own 4-byte header.  Each shape is one zlib stream:
+// u32 flags, (version 4) a NUL-terminated name, u64 vertex and triangle
+// counts, positions, then normals, texture coordinates and colours as the
+// flags say (FP64 when flag 0x2000 is set, else FP32; colours are skipped),
+// then u32 x 3 indices per triangle.  Texture coordinates are not flipped.
+Mesh load_serialized(const std::string &file, int shape_index, const M4 &to_world) {
+    std::ifstream in(file, std::ios::binary);
+    if (!in) fail(LJX_ERR_INVALID_ARGUMENT, "cannot open .serialized file " + file);
+    std::vector<unsigned char> buf((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    auto rd = [&](size_t at, void *dst, size_t n) {
+        if (at + n > buf.size()) fail(LJX_ERR_INVALID_ARGUMENT, "truncated .serialized file " + file);
+        std::memcpy(dst, buf.data() + at, n);
+    };
+    uint16_t version = 0;
+    rd(2, &version, 2);
+    if (version != 3 && version != 4) fail(LJX_ERR_INVALID_ARGUMENT, "unknown .serialized version in " + file);
+    size_t start = 4;
+    if (shape_index > 0) {
+        uint32_t count = 0;
+        rd(buf.size() - 4, &count, 4);
+        if (uint32_t(shape_index) >= count)
+            fail(LJX_ERR_INVALID_ARGUMENT, "shapeIndex " + std::to_string(shape_index) + " out of range in " + file);
+        if (version == 4) {
+            uint64_t off = 0;
+            rd(buf.size() - 4 - 8 * size_t(count - shape_index), &off, 8);
+            start = size_t(off) + 4;
+        } else {
+            uint32_t off = 0;
+            rd(buf.size() - 4 * size_t(count - shape_index + 1), &off, 4);
+            start = size_t(off) + 4;
+        }
+        if (start > buf.size()) fail(LJX_ERR_INVALID_ARGUMENT, "bad shape offset in " + file);
+    }
+    // inflate the shape's stream
+    std::vector<unsigned char> z;
+    z_stream zs{};
+    if (inflateInit2(&zs, 15) != Z_OK) fail(LJX_ERR_INVALID_ARGUMENT, "zlib initialisation failed");
+    zs.next_in = buf.data() + start;
+    zs.avail_in = uInt(buf.size() - start);
+    unsigned char chunk[1 << 16];
+    int ret = Z_OK;
+    while (ret != Z_STREAM_END) {
+        zs.next_out = chunk;
+        zs.avail_out = sizeof(chunk);
+        ret = inflate(&zs, Z_NO_FLUSH);
+        if (ret != Z_OK && ret != Z_STREAM_END) {
+            inflateEnd(&zs);
+            fail(LJX_ERR_INVALID_ARGUMENT, "corrupt zlib stream in " + file);
+        }
+        z.insert(z.end(), chunk, chunk + (sizeof(chunk) - zs.avail_out));
+        if (ret == Z_OK && zs.avail_in == 0 && zs.avail_out != 0) break;  // input ended early
+    }
+    inflateEnd(&zs);
+    size_t at = 0;
+    auto take = [&](void *dst, size_t n) {
+        if (at + n > z.size()) fail(LJX_ERR_INVALID_ARGUMENT, "mesh data ends early in " + file);
+        std::memcpy(dst, z.data() + at, n);
+        at += n;
+    };
+    uint32_t flags = 0;
+    take(&flags, 4);
+    if (version == 4) {
+        char c = 1;
+        while (c != 0) take(&c, 1);
+    }
+    uint64_t nv = 0, nt = 0;
+    take(&nv, 8);
+    take(&nt, 8);
+    if (nv > (1ull << 31) || nt > (1ull << 31)) fail(LJX_ERR_INVALID_ARGUMENT, "mesh too large in " + file);
+    const bool dbl = flags & 0x2000;
+    auto real = [&]() {
+        if (dbl) {
+            double v;
+            take(&v, 8);
+            return v;
+        }
+        float v;
+        take(&v, 4);
+        return double(v);
+    };
+    M4 inv;
+    if (!invert(to_world, inv)) fail(LJX_ERR_INVALID_ARGUMENT, "singular toWorld transform");
+    Mesh m;
+    m.pos.resize(nv);
+    for (auto &p : m.pos) {
+        double x = real(), y = real(), zz = real();
+        p = xform_point(to_world, {x, y, zz});
+    }
+    if (flags & 0x1) {
+        m.nrm.resize(nv);
+        for (auto &v : m.nrm) {
+            double x = real(), y = real(), zz = real();
+            v = xform_normal(inv, {x, y, zz});
+        }
+    }
+    if (flags & 0x2) {
+        m.uv.resize(nv);
+        for (auto &t : m.uv) {
+            double u = real(), v = real();
+            t = {u, v};
+        }
+    }
+    if (flags & 0x8)
+        for (size_t k = 0; k < 3 * nv; k++) real();
+    m.idx.resize(3 * nt);
+    for (auto &i : m.idx) {
+        uint32_t v;
+        take(&v, 4);
+        if (v >= nv) fail(LJX_ERR_INVALID_ARGUMENT, "vertex index out of range in " + file);
+        i = int(v);
+    }
+    return m;
+}
+
+// ------------------------------------------------------------------- PLY
+// Stanford PLY meshes (parse_ply.cpp:9-123): ascii, binary_little_endian or
+// binary_big_endian; vertex x y z (required), u v and nx ny nz when present,
+// face vertex_indices lists; other elements and properties are skipped.
+// Positions and normals take the toWorld transform, texture coordinates are
+// not flipped.  Faces with more than three corners are split into fans.
+Mesh load_ply(const std::string &file, const M4 &to_world) {
+    std::ifstream in(file, std::ios::binary);
+    if (!in) fail(LJX_ERR_INVALID_ARGUMENT, "cannot open PLY file " + file);
+    struct Prop {
+        std::string name;
+        int type = 0;       // size in bytes; negative for signed ints; 4f/8f below
+        bool fp = false;
+        bool list = false;
+        int count_type = 0;
+        bool count_fp = false;
+    };
+    struct Elem {
+        std::string name;
+        size_t count = 0;
+        std::vector<Prop> props;
+    };
+    auto type_of = [&](const std::string &t, int &size, bool &fp) {
+        fp = false;
+        if (t == "char" || t == "int8") size = -1;
+        else if (t == "uchar" || t == "uint8") size = 1;
+        else if (t == "short" || t == "int16") size = -2;
+        else if (t == "ushort" || t == "uint16") size = 2;
+        else if (t == "int" || t == "int32") size = -4;
+        else if (t == "uint" || t == "uint32") size = 4;
+        else if (t == "float" || t == "float32") size = 4, fp = true;
+        else if (t == "double" || t == "float64") size = 8, fp = true;
+        else fail(LJX_ERR_INVALID_ARGUMENT, "unknown PLY property type " + t + " in " + file);
+    };
+    std::string line;
+    std::getline(in, line);
+    if (line.rfind("ply", 0) != 0) fail(LJX_ERR_INVALID_ARGUMENT, "not a PLY file: " + file);
+    int format = -1;  // 0 ascii, 1 little endian, 2 big endian
+    std::vector<Elem> elems;
+    while (std::getline(in, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        std::istringstream ls(line);
+        std::string kw;
+        ls >> kw;
+        if (kw == "format") {
+            std::string f;
+            ls >> f;
+            format = f == "ascii" ? 0 : f == "binary_little_endian" ? 1 : f == "binary_big_endian" ? 2 : -1;
+            if (format < 0) fail(LJX_ERR_INVALID_ARGUMENT, "unknown PLY format " + f + " in " + file);
+        } else if (kw == "element") {
+            Elem e;
+            ls >> e.name >> e.count;
+            elems.push_back(e);
+        } else if (kw == "property") {
+            if (elems.empty()) fail(LJX_ERR_INVALID_ARGUMENT, "PLY property before any element in " + file);
+            Prop p;
+            std::string t;
+            ls >> t;
+            if (t == "list") {
+                std::string ct, it;
+                ls >> ct >> it >> p.name;
+                p.list = true;
+                type_of(ct, p.count_type, p.count_fp);
+                type_of(it, p.type, p.fp);
+            } else {
+                ls >> p.name;
+                type_of(t, p.type, p.fp);
+            }
+            elems.back().props.push_back(p);
+        } else if (kw == "end_header") {
+            break;
+        }
+    }
+    if (format < 0) fail(LJX_ERR_INVALID_ARGUMENT, "PLY file without a format line: " + file);
+    const bool swap = format == 2;
+    auto read_bin = [&](int type, bool fp) -> double {
+        unsigned char b[8];
+        int n = std::abs(type);
+        in.read(reinterpret_cast<char *>(b), n);
+        if (!in) fail(LJX_ERR_INVALID_ARGUMENT, "PLY data ends early in " + file);
+        if (swap) std::reverse(b, b + n);
+        if (fp) {
+            if (n == 4) {
+                float v;
+                std::memcpy(&v, b, 4);
+                return v;
+            }
+            double v;
+            std::memcpy(&v, b, 8);
+            return v;
+        }
+        switch (type) {
+        case -1: return double(int8_t(b[0]));
+        case 1: return double(b[0]);
+        case -2: { int16_t v; std::memcpy(&v, b, 2); return v; }
+        case 2: { uint16_t v; std::memcpy(&v, b, 2); return v; }
+        case -4: { int32_t v; std::memcpy(&v, b, 4); return v; }
+        default: { uint32_t v; std::memcpy(&v, b, 4); return v; }
+        }
+    };
+    auto read_ascii = [&]() -> double {
+        std::string tok;
+        if (!(in >> tok)) fail(LJX_ERR_INVALID_ARGUMENT, "PLY data ends early in " + file);
+        return std::strtod(tok.c_str(), nullptr);
+    };
+    auto value = [&](int type, bool fp) { return format == 0 ? read_ascii() : read_bin(type, fp); };
+    M4 inv;
+    if (!invert(to_world, inv)) fail(LJX_ERR_INVALID_ARGUMENT, "singular toWorld transform");
+    Mesh m;
+    bool have_pos = false, have_faces = false;
+    for (const Elem &e : elems) {
+        const bool vert = e.name == "vertex", face = e.name == "face";
+        int ix = -1, iy = -1, iz = -1, inx = -1, iny = -1, inz = -1, iu = -1, iv = -1, ifc = -1;
+        for (int k = 0; k < int(e.props.size()); k++) {
+            const std::string &n = e.props[k].name;
+            if (vert && !e.props[k].list) {
+                if (n == "x") ix = k;
+                else if (n == "y") iy = k;
+                else if (n == "z") iz = k;
+                else if (n == "nx") inx = k;
+                else if (n == "ny") iny = k;
+                else if (n == "nz") inz = k;
+                else if (n == "u") iu = k;
+                else if (n == "v") iv = k;
+            }
+            if (face && e.props[k].list && n == "vertex_indices") ifc = k;
+        }
+        const bool pos = ix >= 0 && iy >= 0 && iz >= 0, nrm = inx >= 0 && iny >= 0 && inz >= 0, uv = iu >= 0 && iv >= 0;
+        if (vert) have_pos = pos;
+        if (face) have_faces = ifc >= 0;
+        std::vector<double> vals(e.props.size());
+        std::vector<int> corners;
+        for (size_t r = 0; r < e.count; r++) {
+            corners.clear();
+            for (int k = 0; k < int(e.props.size()); k++) {
+                const Prop &p = e.props[k];
+                if (p.list) {
+                    const double cnt = value(p.count_type, p.count_fp);
+                    if (cnt < 0 || cnt > 1e6) fail(LJX_ERR_INVALID_ARGUMENT, "bad PLY list length in " + file);
+                    for (int j = 0; j < int(cnt); j++) {
+                        double v = value(p.type, p.fp);
+                        if (k == ifc) corners.push_back(int(v));
+                    }
+                } else {
+                    vals[k] = value(p.type, p.fp);
+                }
+            }
+            if (vert && pos) {
+                m.pos.push_back(xform_point(to_world, {vals[ix], vals[iy], vals[iz]}));
+                if (nrm) m.nrm.push_back(xform_normal(inv, {vals[inx], vals[iny], vals[inz]}));
+                if (uv) m.uv.push_back({vals[iu], vals[iv]});
+            }
+            if (face && ifc >= 0)
+                for (size_t j = 2; j < corners.size(); j++)
+                    m.idx.insert(m.idx.end(), {corners[0], corners[j - 1], corners[j]});
+        }
+    }
+    if (!have_pos) fail(LJX_ERR_INVALID_ARGUMENT, "vertex positions not found in " + file);
+    if (!have_faces) fail(LJX_ERR_INVALID_ARGUMENT, "vertex indices not found in " + file);
+    for (int i : m.idx)
+        if (i < 0 || size_t(i) >= m.pos.size()) fail(LJX_ERR_INVALID_ARGUMENT, "vertex index out of range in " + file);
     return m;
 }
 
@@ -1060,7 +1354,18 @@ class SceneReader {
             for (auto &v : m.nrm) v = xform_normal(inv, v);
             add_mesh(s, m);
         } else if (t == "serialized" || t == "ply") {
-            fail(LJX_ERR_UNSUPPORTED, "." + t + " meshes are not supported by this build");
+            std::string file;
+            int index = 0;
+            for (const XNode &c : n.kids) {
+                std::string nm = c.get("name");
+                if (nm == "filename") file = resolve(c.get("value"), defaults_);
+                else if (nm == "shapeIndex" || nm == "shape_index") index = integer(c.get("value"), defaults_);
+            }
+            Mesh m = t == "ply" ? load_ply(path(file), to_world_of())
+                                : load_serialized(path(file), index, to_world_of());
+            if (flag("faceNormals", "face_normals")) m.nrm.clear();
+            else if (m.nrm.empty()) m.nrm = vertex_normals(m);
+            add_mesh(s, m);
         } else {
             fail(LJX_ERR_INVALID_ARGUMENT, "unknown shape: " + t);
         }

@@ -176,7 +176,8 @@ def test_media_points_directional_and_volpath(tmp_path):
 
 @pytest.mark.parametrize("body,code", [
     ('<texture type="bitmap" id="t"/>', capi.ERR_UNSUPPORTED),
-    ('<shape type="ply"><string name="filename" value="x.ply"/></shape>', capi.ERR_UNSUPPORTED),
+    ('<shape type="ply"><string name="filename" value="x.ply"/></shape>', capi.ERR_INVALID_ARGUMENT),
+    ('<shape type="serialized"><string name="filename" value="x.serialized"/></shape>', capi.ERR_INVALID_ARGUMENT),
     ('<emitter type="envmap"><string name="filename" value="sky.exr"/></emitter>', capi.ERR_UNSUPPORTED),
     ('<medium type="cloudy" id="m"/>', capi.ERR_INVALID_ARGUMENT),
     ('<medium type="heterogeneous" id="m"><volume name="density" type="gridvolume"/></medium>',
