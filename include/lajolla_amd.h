@@ -75,6 +75,11 @@ enum { LJX_LIGHT_DIFFUSE_AREA = 0, LJX_LIGHT_ENVMAP = 1 };
 /* Medium = variant<HomogeneousMedium, HeterogeneousMedium> src/medium.h:22 */
 enum { LJX_MEDIUM_HOMOGENEOUS = 0, LJX_MEDIUM_HETEROGENEOUS = 1 };
 
+/* Texture<T> = variant<ConstantTexture, ImageTexture, CheckerboardTexture>
+ *                                                       src/texture.h:113 */
+enum { LJX_TEXTURE_IMAGE = 1, LJX_TEXTURE_CHECKERBOARD = 2 };
+#define LJX_MAT_TEXTURE_SLOTS 13
+
 /* Volume<T> = variant<ConstantVolume<T>, GridVolume<T>>   src/volume.h:13-31 */
 enum { LJX_VOLUME_CONSTANT = 0, LJX_VOLUME_GRID = 1 };
 
@@ -143,11 +148,20 @@ typedef struct ljx_material_desc {
     double sheen_tint;
     double clearcoat;
     double clearcoat_gloss;
+    /* Texture<T> fields (src/material.h:12-98): 0 takes the constant above;
+     * k > 0 takes textures[k - 1] of the scene, in the order color_a,
+     * color_b, roughness, specular_transmission, metallic, subsurface,
+     * specular, specular_tint, anisotropic, sheen, sheen_tint, clearcoat,
+     * clearcoat_gloss.  A scalar field reads a 1-channel image as is and the
+     * average of a 3-channel image or of the checkerboard colours. */
+    int32_t texture[LJX_MAT_TEXTURE_SLOTS];
 } ljx_material_desc;
 
 /* One light (src/light.h:12-30).  An environment map is an image of
- * env_width x env_height RGB texels (row-major, float); this build accepts
- * the 1x1 case, which is how a constant environment is expressed. */
+ * env_width x env_height RGB texels (row-major from the top row, float) in
+ * lat-long layout (src/lights/envmap.inl), sampled by a TableDist2D over
+ * luminance x sin(elevation) at texel centres; a 1x1 image is a constant
+ * environment. */
 typedef struct ljx_light_desc {
     uint32_t type;                 /* LJX_LIGHT_* */
     int32_t shape_id;              /* diffuse area light: emitting shape */
@@ -215,6 +229,24 @@ typedef struct ljx_render_options {
     uint64_t seed;                 /* mixed into every (pixel, sample) stream */
 } ljx_render_options;
 
+/* An image or checkerboard texture (src/texture.h:82-154).  An image is
+ * width x height pixels of channels floats (1 or 3), row-major from the top
+ * row; ljx_scene_create builds its mipmap (2x2 box filter, at most 8 levels,
+ * src/mipmap.h:25-48).  A lookup at uv reads the image at
+ * (uv * scale + offset) mod 1, bilinearly on the level the hit's footprint
+ * selects (trilinear between levels), or the checkerboard colour of the
+ * half-cell (src/texture.h:141-154). */
+typedef struct ljx_texture_desc {
+    uint32_t type;                 /* LJX_TEXTURE_* */
+    int32_t channels;              /* image: 1 or 3 */
+    int32_t width;                 /* image */
+    int32_t height;
+    const float *data;             /* image pixels; the caller owns them */
+    double color0[3];              /* checkerboard */
+    double color1[3];
+    double uscale, vscale, uoffset, voffset;
+} ljx_texture_desc;
+
 typedef struct ljx_scene_desc {
     ljx_camera_desc camera;
     const ljx_shape_desc *shapes;
@@ -227,6 +259,8 @@ typedef struct ljx_scene_desc {
     int32_t num_media;
     int32_t envmap_light_id;       /* -1 if none (src/scene.h:66) */
     ljx_render_options options;
+    const ljx_texture_desc *textures;  /* TexturePool (src/texture.h:13-19) */
+    int32_t num_textures;
 } ljx_scene_desc;
 
 /* ---- rays and hits for the intersection boundary ------------------------ */

@@ -78,6 +78,29 @@ class MaterialDesc(C.Structure):
         ("sheen_tint", C.c_double),
         ("clearcoat", C.c_double),
         ("clearcoat_gloss", C.c_double),
+        ("texture", C.c_int32 * 13),
+    ]
+
+
+# texture slots of ljx_material_desc.texture, in order
+TEXTURE_SLOTS = ("color_a", "color_b", "roughness", "specular_transmission", "metallic", "subsurface", "specular",
+                 "specular_tint", "anisotropic", "sheen", "sheen_tint", "clearcoat", "clearcoat_gloss")
+TEXTURE_IMAGE, TEXTURE_CHECKERBOARD = 1, 2
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [
+        ("type", C.c_uint32),
+        ("channels", C.c_int32),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("data", C.POINTER(C.c_float)),
+        ("color0", D3),
+        ("color1", D3),
+        ("uscale", C.c_double),
+        ("vscale", C.c_double),
+        ("uoffset", C.c_double),
+        ("voffset", C.c_double),
     ]
 
 
@@ -157,6 +180,8 @@ class SceneDescC(C.Structure):
         ("num_media", C.c_int32),
         ("envmap_light_id", C.c_int32),
         ("options", RenderOptions),
+        ("textures", C.POINTER(TextureDesc)),
+        ("num_textures", C.c_int32),
     ]
 
 

@@ -115,11 +115,26 @@ struct Rng {
 // Material record: constant-texture parameters in FP64 (src/material.h:12-98).
 struct MatRec {
     int type;
-    int pad;
+    int textured;  // some field reads a texture
     D3 a, b;
     double roughness, eta;
     double specular_transmission, metallic, subsurface, specular, specular_tint, anisotropic, sheen,
         sheen_tint, clearcoat, clearcoat_gloss;
+    int tex[13];  // per field (ljx_material_desc order): 0 the constant, k > 0 texture k - 1
+};
+// ImageTexture / CheckerboardTexture (src/texture.h:82-97) with the image's
+// mipmap (src/mipmap.h:7-48): level l is w[l] x h[l] pixels of channels
+// doubles at tex_data + off[l], row-major from the top row.
+constexpr int kMaxMipLevels = 8;
+struct TexRec {
+    int type;  // 1 image, 2 checkerboard
+    int channels;
+    int levels;
+    int pad;
+    int w[kMaxMipLevels], h[kMaxMipLevels];
+    long long off[kMaxMipLevels];
+    D3 color0, color1;
+    double uscale, vscale, uoffset, voffset;
 };
 // Shape record (src/shape.h:20-51).
 struct ShapeRec {
@@ -137,14 +152,22 @@ struct ShapeRec {
     double radius;
     double total_area;
 };
-// Light record (src/light.h:12-30); envmaps are 1x1 images.
+// Light record (src/light.h:12-30).  An envmap reads texture env_tex and
+// samples directions from a TableDist2D over its texels (src/table_dist.cpp:40-151).
 struct LightRec {
     int type;  // 0 area, 1 envmap
     int shape_id;
     D3 intensity;
-    D3 env_value;
     double to_world[9], to_local[9];  // 3x3 (vectors only)
     double scale;
+    int env_tex;
+    int env_w, env_h;
+    int pad;
+    const float *env_rgb;         // level 0 of the envmap's texture, FP32 as given
+    const double *cdf_rows;       // env_h x (env_w + 1)
+    const double *pdf_rows;       // env_h x env_w
+    const double *cdf_marginals;  // env_h + 1
+    const double *pdf_marginals;  // env_h
 };
 // A spectrum-valued volume (src/volume.h:13-31): a constant, or a grid of
 // FP32 voxels (1 channel: grey, 3: RGB; x fastest) over a box, read with
@@ -215,6 +238,8 @@ struct DeviceScene {
     const MatRec *materials;
     const LightRec *lights;
     const MediumRec *media;
+    const TexRec *textures;
+    const double *tex_data;   // every texture level's pixels
     const double *light_pmf;  // num_lights
     const double *light_cdf;  // num_lights + 1
     const double *tri_cdf;    // concatenated per-light-shape triangle CDFs
@@ -226,6 +251,59 @@ struct DeviceScene {
     double bound_radius;
     double eps;  // min(1e-5 * r, 0.01) (src/scene.h:99-105)
 };
+
+// ---------------------------------------------------------------- textures
+// src/lajolla.h:48-61
+LJX_HD int imod(int a, int b) {
+    int r = a % b;
+    return r < 0 ? r + b : r;
+}
+LJX_HD double rmod(double a, double b) {
+    double r = fmod(a, b);
+    return r < 0 ? r + b : r;
+}
+LJX_HD D3 texel(const double *data, const TexRec &t, int level, int x, int y) {
+    const double *p = data + t.off[level] + size_t(t.channels) * (size_t(y) * t.w[level] + x);
+    return t.channels == 1 ? D3{p[0], p[0], p[0]} : D3{p[0], p[1], p[2]};
+}
+// src/mipmap.h:50-72: bilinear on one level, pixel centres at half-integers
+LJX_HD D3 mip_lookup(const double *data, const TexRec &t, double u, double v, int level) {
+    const int w = t.w[level], h = t.h[level];
+    u = u * w - 0.5;
+    v = v * h - 0.5;
+    const int ufi = imod(int(u), w), vfi = imod(int(v), h);
+    const int uci = imod(ufi + 1, w), vci = imod(vfi + 1, h);
+    const double uo = u - ufi, vo = v - vfi;
+    return texel(data, t, level, ufi, vfi) * (1 - uo) * (1 - vo) + texel(data, t, level, ufi, vci) * (1 - uo) * vo +
+           texel(data, t, level, uci, vfi) * uo * (1 - vo) + texel(data, t, level, uci, vci) * uo * vo;
+}
+// src/mipmap.h:74-88: trilinear between levels
+LJX_HD D3 mip_lookup_level(const double *data, const TexRec &t, double u, double v, double level) {
+    const int n = t.levels;
+    if (level <= 0) return mip_lookup(data, t, u, v, 0);
+    if (level < double(n - 1)) {
+        const int fl = min(max(int(floor(level)), 0), n - 1), cl = min(max(fl + 1, 0), n - 1);
+        const double lo = level - fl;
+        return mip_lookup(data, t, u, v, fl) * (1 - lo) + mip_lookup(data, t, u, v, cl) * lo;
+    }
+    return mip_lookup(data, t, u, v, n - 1);
+}
+// src/texture.h:131-154
+LJX_HD D3 tex_eval(const double *data, const TexRec &t, double u, double v, double footprint) {
+    const double lu = rmod(u * t.uscale + t.uoffset, 1.0), lv = rmod(v * t.vscale + t.voffset, 1.0);
+    if (t.type == 2) {
+        const int x = 2 * imod(int(lu * 2), 2) - 1, y = 2 * imod(int(lv * 2), 2) - 1;
+        return x * y == 1 ? t.color0 : t.color1;
+    }
+    const double scaled = max(t.w[0], t.h[0]) * fmax(t.uscale, t.vscale) * footprint;
+    const double level = log2(scaled < double(1e-8f) ? double(1e-8f) : scaled);  // std::max(scaled, 1e-8f)
+    return mip_lookup_level(data, t, lu, lv, level);
+}
+// a Texture<Real> field: a 1-channel image as is, otherwise the average
+LJX_HD double tex_eval1(const double *data, const TexRec &t, double u, double v, double footprint) {
+    D3 c = tex_eval(data, t, u, v, footprint);
+    return (t.type == 1 && t.channels == 1) ? c.x : (c.x + c.y + c.z) / 3;
+}
 
 // ---------------------------------------------------------------- sampling
 // TableDist1D sampling: upper_bound over the CDF, clamped (src/table_dist.cpp:27-33).

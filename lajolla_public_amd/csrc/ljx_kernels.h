@@ -229,7 +229,9 @@ __global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int c
                 P.hit[pid] = make_float4(h.t, h.u, h.v, __int_as_float(h.slot));
                 int gid = __float_as_int(S.prims[3 * h.slot].w);
                 int mid = S.shapes[S.prim_shape[gid]].material_id;
-                bucket = mid < 0 ? kBucketNoMat : (sorted ? int(S.materials[mid].type) : kBucketAny);
+                // textured materials are shaded by the runtime-dispatch kernel, which evaluates them
+                bucket = mid < 0 ? kBucketNoMat
+                                 : (sorted && !S.materials[mid].textured ? int(S.materials[mid].type) : kBucketAny);
             } else {
                 P.hit[pid] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
                 bucket = kBucketMiss;
@@ -271,7 +273,22 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
     float4 rad4 = P.rad[pid];
     D3 rad{rad4.x, rad4.y, rad4.z};
     Vertex v = make_vertex<SPH>(S, __float_as_int(h.w), h.x, h.y, h.z, org, dir);
-    const MatRec &mat = S.materials[v.material_id];
+    // constant materials are read in place; the runtime-dispatch kernel, which
+    // also takes textured ones, works on a copy with the textures evaluated
+    // (the footprint is nonzero on camera rays only, path_tracing.h:14-16)
+    MatRec mat_tex;
+    if constexpr (MT < 0) {
+        mat_tex = S.materials[v.material_id];
+        if (mat_tex.textured) {
+            double fp = 0;
+            if (b == 0) {
+                const SlotInfo si = slot_info(P, int(pid));
+                fp = (0.25 / double(max(si.W, si.H))) * sqrt(dist2(org, v.pos)) / v.inv_uv;
+            }
+            resolve_textures(S, mat_tex, v.tu, v.tv, fp);
+        }
+    }
+    const MatRec &mat = MT < 0 ? mat_tex : S.materials[v.material_id];
     bool alive = true;
     push_ext = false;
     push_sh = false;
@@ -291,7 +308,7 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
     if (b == 0) {
         // path_tracing.h:58-61
         if (v.area_light_id >= 0)
-            rad = rad + tp * light_emission(S.lights[v.area_light_id], -dir, LightPoint{v.pos, v.g.ng});
+            rad = rad + tp * light_emission(S, S.lights[v.area_light_id], -dir, LightPoint{v.pos, v.g.ng});
     } else {
         // path_tracing.h:243-322 for the vertex reached by the previous sample
         float4 pv = P.prev[pid];
@@ -302,7 +319,7 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
         if (v.area_light_id >= 0) {
             const LightRec &l = S.lights[v.area_light_id];
             LightPoint lp{v.pos, v.g.ng};
-            D3 L = light_emission(l, -dir, lp);
+            D3 L = light_emission(S, l, -dir, lp);
             D3 C2 = G * f * L;
             double p1 = S.light_pmf[v.area_light_id] * pdf_light_point(S, l, lp, org);
             double w2 = (p2 * p2) / (p1 * p1 + p2 * p2);
@@ -348,7 +365,7 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
         D3 dir_view = -dir;
         if (G > 0 && p1 > 0) {
             D3 f = bsdf_eval<MT>(mat, dir_view, dir_light, v.g);
-            D3 L = light_emission(l, -dir_light, lp);
+            D3 L = light_emission(S, l, -dir_light, lp);
             D3 C1 = G * f * L;
             double pb = bsdf_pdf<MT>(mat, dir_view, dir_light, v.g);
             if (guided)
@@ -455,7 +472,7 @@ __global__ __launch_bounds__(256) void k_miss(DeviceScene S, PathBufs P) {
         D3 tp{tp4.x, tp4.y, tp4.z};
         float4 r4 = P.rad[pid];
         D3 rad{r4.x, r4.y, r4.z};
-        D3 L = light_emission(env, -dir, LightPoint{{0, 0, 0}, {0, 0, 0}});
+        D3 L = light_emission(S, env, -dir, LightPoint{{0, 0, 0}, {0, 0, 0}});
         D3 rad_in = rad;
         int b = P.depth[pid];
         if (b == 0) {
@@ -605,6 +622,17 @@ __global__ __launch_bounds__(256) void k_aux(DeviceScene S, PathBufs P, int mode
                 c = D3{k, k, k};
             } else if (mode == 3) {
                 c = D3{0, 0.25 / double(max(P.W, P.H)), 0};  // init_ray_differential (src/ray.h:35-37)
+            } else if (mode == 4 && v.material_id >= 0) {
+                // the level the camera ray's footprint selects in the material's first colour
+                // texture when it is an image (render.cpp:44-60; get_texture, material.cpp:121-123)
+                const MatRec &m = S.materials[v.material_id];
+                if (m.type != 6 && m.tex[0] && S.textures[m.tex[0] - 1].type == 1) {
+                    const TexRec &t = S.textures[m.tex[0] - 1];
+                    double fp = (0.25 / double(max(P.W, P.H))) * sqrt(dist2(v.pos, o)) / v.inv_uv;
+                    double sf = max(t.w[0], t.h[0]) * fmax(t.uscale, t.vscale) * fp;
+                    double level = log2(sf < double(1e-8f) ? double(1e-8f) : sf);
+                    c = D3{level, level, level};
+                }
             }
         }
         out[3 * size_t(i)] = float(c.x);

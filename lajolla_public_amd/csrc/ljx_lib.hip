@@ -70,6 +70,81 @@ bool mat_inv4(const double *m, double *out) {
 double luminance_h(const double *c) { return c[0] * 0.212671 + c[1] * 0.715160 + c[2] * 0.072169; }
 
 // TableDist1D construction (src/table_dist.cpp:3-24).
+// make_mipmap (src/mipmap.h:25-48): 2x2 box filter in FP64, at most 8
+// levels.  Where a level's width or height is already 1 the reference reads
+// the pixel past the row or column (outside the image on its last row); that
+// read is clamped to the image here.
+void build_mipmap(TexRec &t, std::vector<double> &data, const float *px, int w, int h, int ch) {
+    t.w[0] = w;
+    t.h[0] = h;
+    t.off[0] = (long long)data.size();
+    for (size_t i = 0; i < size_t(w) * h * ch; i++) data.push_back(px[i]);
+    int num_levels = std::min(int(std::ceil(std::log2(double(std::max(w, h))) + 1)), kMaxMipLevels);
+    t.levels = num_levels;
+    for (int l = 1; l < num_levels; l++) {
+        const int pw = t.w[l - 1], ph = t.h[l - 1];
+        const int nw = std::max(pw / 2, 1), nh = std::max(ph / 2, 1);
+        const long long po = t.off[l - 1];
+        t.w[l] = nw;
+        t.h[l] = nh;
+        t.off[l] = (long long)data.size();
+        data.resize(data.size() + size_t(nw) * nh * ch);
+        for (int y = 0; y < nh; y++)
+            for (int x = 0; x < nw; x++)
+                for (int c = 0; c < ch; c++) {
+                    auto at = [&](int xx, int yy) {
+                        return data[po + size_t(ch) * (size_t(std::min(yy, ph - 1)) * pw + std::min(xx, pw - 1)) + c];
+                    };
+                    data[t.off[l] + size_t(ch) * (size_t(y) * nw + x) + c] =
+                        (at(2 * x, 2 * y) + at(2 * x + 1, 2 * y) + at(2 * x, 2 * y + 1) + at(2 * x + 1, 2 * y + 1)) / 4.0;
+                }
+    }
+}
+
+// make_table_dist_2d (src/table_dist.cpp:40-95)
+struct Dist2D {
+    std::vector<double> cdf_rows, pdf_rows, cdf_marginals, pdf_marginals;
+    double total = 0;
+};
+Dist2D make_dist_2d(const std::vector<double> &f, int w, int h) {
+    Dist2D t;
+    t.cdf_rows.assign(size_t(h) * (w + 1), 0);
+    t.pdf_rows.assign(size_t(h) * w, 0);
+    for (int y = 0; y < h; y++) {
+        double *c = &t.cdf_rows[size_t(y) * (w + 1)];
+        c[0] = 0;
+        for (int x = 0; x < w; x++) c[x + 1] = c[x] + f[size_t(y) * w + x];
+        double integral = c[w];
+        if (integral > 0) {
+            for (int x = 0; x < w; x++) c[x] /= integral;
+            for (int x = 0; x < w; x++) t.pdf_rows[size_t(y) * w + x] = f[size_t(y) * w + x] / integral;
+        } else {
+            for (int x = 0; x < w; x++) {
+                t.pdf_rows[size_t(y) * w + x] = 1.0 / double(w);
+                c[x] = double(x) / double(w);
+            }
+            c[w] = 1;
+        }
+    }
+    t.cdf_marginals.assign(h + 1, 0);
+    t.pdf_marginals.assign(h, 0);
+    for (int y = 0; y < h; y++) t.cdf_marginals[y + 1] = t.cdf_marginals[y] + t.cdf_rows[size_t(y) * (w + 1) + w];
+    t.total = t.cdf_marginals[h];
+    if (t.total > 0) {
+        for (int y = 0; y < h; y++) t.cdf_marginals[y] /= t.total;
+        t.cdf_marginals[h] = 1;
+        for (int y = 0; y < h; y++) t.pdf_marginals[y] = t.cdf_rows[size_t(y) * (w + 1) + w] / t.total;
+    } else {
+        for (int y = 0; y < h; y++) {
+            t.pdf_marginals[y] = 1.0 / double(h);
+            t.cdf_marginals[y] = double(y) / double(h);
+        }
+        t.cdf_marginals[h] = 1;
+    }
+    for (int y = 0; y < h; y++) t.cdf_rows[size_t(y) * (w + 1) + w] = 1;
+    return t;
+}
+
 void make_dist(const std::vector<double> &f, std::vector<double> &pmf, std::vector<double> &cdf) {
     size_t n = f.size();
     pmf = f;
@@ -125,6 +200,7 @@ struct Scene {
     int num_materials = 0;
     bool has_spheres = false;
     unsigned mat_types_present = 0;
+    bool has_textured = false;  // some material reads a texture: shaded by the runtime-dispatch kernel
     bool has_media = false;
     int num_media = 0;
     double bound_center[3] = {0, 0, 0};
@@ -206,8 +282,24 @@ void validate(const ljx_scene_desc *d) {
         (d->num_media && !d->media))
         fail(LJX_ERR_INVALID_ARGUMENT, "NULL array with a nonzero count");
     if (c.medium_id < -1 || c.medium_id >= d->num_media) fail(LJX_ERR_INVALID_ARGUMENT, "camera medium id out of range");
-    for (int i = 0; i < d->num_materials; i++)
+    if (d->num_textures < 0 || (d->num_textures > 0 && !d->textures))
+        fail(LJX_ERR_INVALID_ARGUMENT, "NULL texture array with a nonzero count");
+    for (int i = 0; i < d->num_textures; i++) {
+        const ljx_texture_desc &t = d->textures[i];
+        if (t.type == LJX_TEXTURE_IMAGE) {
+            if (!t.data || t.width < 1 || t.height < 1 || (t.channels != 1 && t.channels != 3) ||
+                double(t.width) * t.height > double(1u << 28))
+                fail(LJX_ERR_INVALID_ARGUMENT, "texture " + std::to_string(i) + ": an image needs pixels, a size and 1 or 3 channels");
+        } else if (t.type != LJX_TEXTURE_CHECKERBOARD) {
+            fail(LJX_ERR_INVALID_ARGUMENT, "texture " + std::to_string(i) + ": unknown type");
+        }
+    }
+    for (int i = 0; i < d->num_materials; i++) {
         if (d->materials[i].type >= LJX_MAT_COUNT) fail(LJX_ERR_INVALID_ARGUMENT, "unknown material type");
+        for (int k = 0; k < LJX_MAT_TEXTURE_SLOTS; k++)
+            if (d->materials[i].texture[k] < 0 || d->materials[i].texture[k] > d->num_textures)
+                fail(LJX_ERR_INVALID_ARGUMENT, "material " + std::to_string(i) + ": texture index out of range");
+    }
     for (int i = 0; i < d->num_media; i++) {
         const ljx_medium_desc &m = d->media[i];
         if (m.type > LJX_MEDIUM_HETEROGENEOUS) fail(LJX_ERR_INVALID_ARGUMENT, "unknown medium type");
@@ -253,8 +345,9 @@ void validate(const ljx_scene_desc *d) {
             if (s.type == LJX_SHAPE_TRIANGLE_MESH && s.num_triangles == 0)
                 fail(LJX_ERR_INVALID_ARGUMENT, "light " + std::to_string(i) + ": emitting mesh has no triangles");
         } else if (l.type == LJX_LIGHT_ENVMAP) {
-            if (l.env_width != 1 || l.env_height != 1 || !l.env_rgb)
-                fail(LJX_ERR_UNSUPPORTED, "only 1x1 (constant) environment maps are supported by this build");
+            if (l.env_width < 1 || l.env_height < 1 || !l.env_rgb ||
+                double(l.env_width) * l.env_height > double(1u << 28))
+                fail(LJX_ERR_INVALID_ARGUMENT, "light " + std::to_string(i) + ": environment map needs texels and a size");
         } else {
             fail(LJX_ERR_INVALID_ARGUMENT, "unknown light type");
         }
@@ -434,8 +527,39 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
         r.sheen_tint = m.sheen_tint;
         r.clearcoat = m.clearcoat;
         r.clearcoat_gloss = m.clearcoat_gloss;
+        for (int k = 0; k < LJX_MAT_TEXTURE_SLOTS; k++) {
+            r.tex[k] = m.texture[k];
+            if (m.texture[k]) r.textured = 1;
+        }
+    }
+    // ---- textures (src/texture.h:13-19): the scene's, then each envmap's image
+    std::vector<TexRec> texs;
+    std::vector<double> tex_data;
+    auto add_image = [&](const float *px, int w, int h, int ch, double us, double vs, double uo, double vo) {
+        TexRec t{};
+        t.type = 1;
+        t.channels = ch;
+        t.uscale = us, t.vscale = vs, t.uoffset = uo, t.voffset = vo;
+        build_mipmap(t, tex_data, px, w, h, ch);
+        texs.push_back(t);
+        return int(texs.size()) - 1;
+    };
+    for (int i = 0; i < d->num_textures; i++) {
+        const ljx_texture_desc &t = d->textures[i];
+        if (t.type == LJX_TEXTURE_IMAGE) {
+            add_image(t.data, t.width, t.height, t.channels, t.uscale, t.vscale, t.uoffset, t.voffset);
+        } else {
+            TexRec r{};
+            r.type = 2;
+            r.color0 = D3{t.color0[0], t.color0[1], t.color0[2]};
+            r.color1 = D3{t.color1[0], t.color1[1], t.color1[2]};
+            r.uscale = t.uscale, r.vscale = t.vscale, r.uoffset = t.uoffset, r.voffset = t.voffset;
+            texs.push_back(r);
+        }
     }
     sc.mat_types_present = 0;
+    sc.has_textured = false;
+    for (int i = 0; i < d->num_materials; i++) sc.has_textured = sc.has_textured || mats[i].textured;
     for (int i = 0; i < d->num_shapes; i++) {
         int mid = d->shapes[i].material_id;
         if (mid >= 0) sc.mat_types_present |= 1u << d->materials[mid].type;
@@ -452,7 +576,11 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
         r.shape_id = l.shape_id;
         r.intensity = D3{l.intensity[0], l.intensity[1], l.intensity[2]};
         if (l.type == LJX_LIGHT_ENVMAP) {
-            r.env_value = D3{l.env_rgb[0], l.env_rgb[1], l.env_rgb[2]};
+            const int w = l.env_width, h = l.env_height;
+            r.env_tex = add_image(l.env_rgb, w, h, 3, 1, 1, 0, 0);
+            r.env_w = w;
+            r.env_h = h;
+            r.env_rgb = sc.upload(std::vector<float>(l.env_rgb, l.env_rgb + size_t(w) * h * 3));
             double inv[16];
             if (!mat_inv4(l.to_world, inv)) fail(LJX_ERR_INVALID_ARGUMENT, "envmap to_world is singular");
             for (int a = 0; a < 3; a++)
@@ -461,9 +589,24 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
                     r.to_local[a * 3 + b] = inv[a * 4 + b];
                 }
             r.scale = l.scale;
-            // light_power: pi r^2 * total / (w h); 1x1 texel: lum * sin(pi / 2)
-            double lum[3] = {r.env_value.x, r.env_value.y, r.env_value.z};
-            power[i] = kPi * sc.bound_radius * sc.bound_radius * (luminance_h(lum) * std::sin(kPi * 0.5));
+            // envmap.inl:75-98: luminance x sin(elevation) at the texel centres of level 0
+            std::vector<double> f(size_t(w) * h);
+            const TexRec &et = texs[r.env_tex];
+            for (int y = 0, k = 0; y < h; y++) {
+                double v = (y + 0.5) / double(h);
+                double se = std::sin(kPi * v);
+                for (int x = 0; x < w; x++) {
+                    double u = (x + 0.5) / double(w);
+                    f[k++] = luminance(mip_lookup(tex_data.data(), et, u, v, 0)) * se;
+                }
+            }
+            Dist2D dd = make_dist_2d(f, w, h);
+            r.cdf_rows = sc.upload(dd.cdf_rows);
+            r.pdf_rows = sc.upload(dd.pdf_rows);
+            r.cdf_marginals = sc.upload(dd.cdf_marginals);
+            r.pdf_marginals = sc.upload(dd.pdf_marginals);
+            // light_power (envmap.inl:1-5): pi r^2 * total / (w h)
+            power[i] = kPi * sc.bound_radius * sc.bound_radius * dd.total / (double(w) * h);
         } else {
             double area = shapes[l.shape_id].total_area;
             power[i] = luminance_h(l.intensity) * area * kPi;
@@ -534,6 +677,8 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
     S.materials = sc.upload(mats);
     S.lights = sc.upload(lights);
     S.media = sc.upload(media);
+    S.textures = sc.upload(texs);
+    S.tex_data = sc.upload(tex_data);
     S.light_pmf = sc.upload(lpmf);
     S.light_cdf = sc.upload(lcdf);
     S.tri_cdf = sc.upload(tri_cdf);
@@ -750,7 +895,8 @@ void shade_all_t(Scene &sc, PathBufs &P, int cur, int grid, int block, hipStream
     if (m & (1u << 6)) launch(k_shade<6, SPH, GUIDE>, grid, block, s, S, P, cur);
     if (m & (1u << 7)) launch(k_shade<7, SPH, GUIDE>, grid, block, s, S, P, cur);
     if (m & (1u << 8)) launch(k_shade<8, SPH, GUIDE>, grid, block, s, S, P, cur);
-    if (P.sort_threshold > 0) launch(k_shade<-1, SPH, GUIDE>, grid, block, s, S, P, cur);
+    // unsorted bounces and textured materials go to the runtime-dispatch kernel
+    if (P.sort_threshold > 0 || sc.has_textured) launch(k_shade<-1, SPH, GUIDE>, grid, block, s, S, P, cur);
     launch(k_miss, grid, block, s, S, P);
 }
 void shade_all(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s, bool guide, int block = 256) {

@@ -29,6 +29,8 @@ __device__ __forceinline__ D3 f3d(float4 v) { return D3{double(v.x), double(v.y)
 struct Vertex {
     D3 pos;
     ShadeGeom g;
+    double tu, tv;   // texture coordinates (src/intersection.h:22)
+    double inv_uv;   // |dp/duv|: the footprint's divisor (intersection.cpp:56-57)
     int shape_id;
     int material_id;
     int area_light_id;
@@ -38,8 +40,8 @@ struct Vertex {
 
 // Rebuilds lajolla's PathVertex for a hit (src/intersection.cpp:35-62 with
 // compute_shading_info, src/shapes/triangle_mesh.inl:77-169 and
-// src/shapes/sphere.inl:243-268).  Ray differentials are not carried: every
-// texture here is constant, so they cannot change a value.
+// src/shapes/sphere.inl:243-268), with the texture coordinates and the
+// size of dp/duv that the footprint of a camera ray divides by.
 template <bool SPH>
 __device__ __forceinline__ Vertex make_vertex(const DeviceScene &S, int slot, float t, float u, float v, D3 org,
                                               D3 dir) {
@@ -76,15 +78,21 @@ __device__ __forceinline__ Vertex make_vertex(const DeviceScene &S, int slot, fl
             u2x = S.vuv[2 * i2];
             u2y = S.vuv[2 * i2 + 1];
         }
+        vx.tu = (1 - s0 - s1) * u0x + s0 * u1x + s1 * u2x;
+        vx.tv = (1 - s0 - s1) * u0y + s0 * u1y + s1 * u2y;
         double dsx = u2x - u0x, dsy = u2y - u0y;  // duv/ds
         double dtx = u2x - u1x, dty = u2y - u1y;  // duv/dt
         double det = dsx * dty - dtx * dsy;
         D3 dpdu;
         if (fabs(det) > double(1e-8f)) {
             double dsdu = dty / det, dtdu = -dsy / det;
+            double dsdv = dtx / det, dtdv = -dsx / det;  // the reference's signs (triangle_mesh.inl:117-120)
             dpdu = (c - a) * dsdu + (c - b) * dtdu;
+            D3 dpdv = (c - a) * dsdv + (c - b) * dtdv;
+            vx.inv_uv = fmax(length(dpdu), length(dpdv));
         } else {
             dpdu = frame_from_n(ng).x;
+            vx.inv_uv = 1;  // coordinate_system: two unit vectors
         }
         D3 n = ng;
         if (sh.has_normals) {
@@ -99,13 +107,20 @@ __device__ __forceinline__ Vertex make_vertex(const DeviceScene &S, int slot, fl
         D3 c{sh.center[0], sh.center[1], sh.center[2]};
         double r = sh.radius;
         double td, su, sv;
-        sphere_hit_uv(make_double4(c.x, c.y, c.z, r), org.x, org.y, org.z, dir.x, dir.y, dir.z, 0.0, t, td, su, sv);
+        // the FP64 root the traversal found, searched for just below its FP32 distance: from 0, a ray
+        // leaving the sphere's own surface (a refraction into it) would find its near root, ~0, instead
+        sphere_hit_uv(make_double4(c.x, c.y, c.z, r), org.x, org.y, org.z, dir.x, dir.y, dir.z,
+                      double(t) * (1 - 1e-6), t, td, su, sv);
         // the reference passes the FP32 distance and u, v through Embree's hit record
         su = double(float(su));
         sv = double(float(sv));
         vx.pos = org + dir * double(float(td));
         ng = normalize(vx.pos - c);
         D3 dpdu{-r * sin(su) * sin(sv), r * cos(su) * sin(sv), 0.0};
+        D3 dpdv{r * cos(su) * cos(sv), r * sin(su) * cos(sv), -r * sin(sv)};
+        vx.tu = su;
+        vx.tv = sv;
+        vx.inv_uv = (length(dpdu) + length(dpdv)) / 2;
         D3 tg = normalize(dpdu - ng * dot(ng, dpdu));
         sf = Frame{tg, normalize(cross(ng, tg)), ng};
     }
@@ -113,6 +128,31 @@ __device__ __forceinline__ Vertex make_vertex(const DeviceScene &S, int slot, fl
     vx.g.ng = ng;
     vx.g.sf = sf;
     return vx;
+}
+
+// The material with its texture fields evaluated at (u, v) with the given
+// footprint, as each BSDF's eval(texture, vertex.uv, vertex.uv_screen_size)
+// calls do (e.g. src/materials/lambertian.inl:11-13).
+__device__ __forceinline__ void resolve_textures(const DeviceScene &S, MatRec &m, double u, double v, double fp) {
+    auto spec = [&](int k, D3 &dst) {
+        if (m.tex[k]) dst = tex_eval(S.tex_data, S.textures[m.tex[k] - 1], u, v, fp);
+    };
+    auto real = [&](int k, double &dst) {
+        if (m.tex[k]) dst = tex_eval1(S.tex_data, S.textures[m.tex[k] - 1], u, v, fp);
+    };
+    spec(0, m.a);
+    spec(1, m.b);
+    real(2, m.roughness);
+    real(3, m.specular_transmission);
+    real(4, m.metallic);
+    real(5, m.subsurface);
+    real(6, m.specular);
+    real(7, m.specular_tint);
+    real(8, m.anisotropic);
+    real(9, m.sheen);
+    real(10, m.sheen_tint);
+    real(11, m.clearcoat);
+    real(12, m.clearcoat_gloss);
 }
 
 // Mean curvature of a hit (compute_shading_info, src/shapes/triangle_mesh.inl:
@@ -210,10 +250,35 @@ __device__ __forceinline__ double pdf_shape(const ShapeRec &sh, const LightPoint
     }
     return 1 / sh.total_area;
 }
-// Envmap on a 1x1 image (src/lights/envmap.inl:11-42): (u, v) uniform,
-// direction from lat-long with y up, pdf = 1 / (2 pi^2 sin(elevation)).
+// TableDist2D sampling and pdf (src/table_dist.cpp:97-151) over the envmap's texels.
+__device__ __forceinline__ void sample_env_uv(const LightRec &l, double rx, double ry, double &u, double &v) {
+    const int w = l.env_w, h = l.env_h;
+    const int yo = sample_cdf(l.cdf_marginals, h, ry);
+    double dy = ry - l.cdf_marginals[yo];
+    if (l.cdf_marginals[yo + 1] - l.cdf_marginals[yo] > 0) dy /= (l.cdf_marginals[yo + 1] - l.cdf_marginals[yo]);
+    const double *c = l.cdf_rows + size_t(yo) * (w + 1);
+    const int xo = sample_cdf(c, w, rx);
+    double dx = rx - c[xo];
+    if (c[xo + 1] - c[xo] > 0) dx /= (c[xo + 1] - c[xo]);
+    u = (xo + dx) / w;
+    v = (yo + dy) / h;
+}
+__device__ __forceinline__ double pdf_env_uv(const LightRec &l, double u, double v) {
+    const int w = l.env_w, h = l.env_h;
+    const int x = int(clampd(u * w, 0, w - 1)), y = int(clampd(v * h, 0, h - 1));
+    return l.pdf_marginals[y] * l.pdf_rows[size_t(y) * w + x] * w * h;
+}
+// lat-long coordinates of a local direction, y up (src/lights/envmap.inl:25-31)
+__device__ __forceinline__ void env_uv(D3 ld, double &u, double &v) {
+    u = atan2(ld.x, -ld.z) * (1 / (2 * kPi));
+    v = acos(clampd(ld.y, -1, 1)) * (1 / kPi);
+    if (u < 0) u += 1;
+}
+// src/lights/envmap.inl:7-42
 __device__ __forceinline__ LightPoint sample_env(const LightRec &l, double u0, double u1) {
-    double az = u0 * (2 * kPi), el = u1 * kPi;
+    double eu, ev;
+    sample_env_uv(l, u0, u1, eu, ev);
+    double az = eu * (2 * kPi), el = ev * kPi;
     double sa, ca, se, ce;
     sincos(az, &sa, &ca);
     sincos(el, &se, &ce);
@@ -223,10 +288,12 @@ __device__ __forceinline__ LightPoint sample_env(const LightRec &l, double u0, d
 }
 __device__ __forceinline__ double pdf_env(const LightRec &l, D3 world_dir) {
     D3 ld = xform3(l.to_local, world_dir);
+    double u, v;
+    env_uv(ld, u, v);
     double ce = ld.y;
     double se = sqrt(clampd(1 - ce * ce, 0, 1));
     if (se <= 0) return 0;
-    return 1.0 / (2 * kPi * kPi * se);
+    return pdf_env_uv(l, u, v) / (2 * kPi * kPi * se);
 }
 __device__ __forceinline__ LightPoint sample_light_point(const DeviceScene &S, const LightRec &l, D3 ref, double u0,
                                                         double u1, double w) {
@@ -238,9 +305,31 @@ __device__ __forceinline__ double pdf_light_point(const DeviceScene &S, const Li
     if (l.type == 1) return pdf_env(l, -lp.n);
     return pdf_shape(S.shapes[l.shape_id], lp, ref);
 }
-// emission (src/lights/diffuse_area_light.inl:15-20; envmap value * scale)
-__device__ __forceinline__ D3 light_emission(const LightRec &l, D3 view_dir, const LightPoint &lp) {
-    if (l.type == 1) return l.env_value * l.scale;
+// emission (src/lights/diffuse_area_light.inl:15-20; src/lights/envmap.inl:44-72).
+// The envmap's footprint, min(|du/dw|, dv/dw), is never positive (dv/dw < 0),
+// so the reference reads the image bilinearly on level 0; only along the exact
+// poles, where that footprint is NaN, would it read the last level instead.
+__device__ __forceinline__ D3 light_emission(const DeviceScene &S, const LightRec &l, D3 view_dir,
+                                            const LightPoint &lp) {
+    if (l.type == 1) {
+        // mip_lookup on level 0, whose texels are the envmap's FP32 values (src/mipmap.h:50-72)
+        D3 w = xform3(l.to_local, -view_dir);
+        double u, v;
+        env_uv(w, u, v);
+        const int W = l.env_w, H = l.env_h;
+        u = u * W - 0.5;
+        v = v * H - 0.5;
+        const int ufi = imod(int(u), W), vfi = imod(int(v), H);
+        const int uci = imod(ufi + 1, W), vci = imod(vfi + 1, H);
+        const double uo = u - ufi, vo = v - vfi;
+        auto tx = [&](int x, int y) {
+            const float *p = l.env_rgb + 3 * (size_t(y) * W + x);
+            return D3{double(p[0]), double(p[1]), double(p[2])};
+        };
+        D3 c = tx(ufi, vfi) * (1 - uo) * (1 - vo) + tx(ufi, vci) * (1 - uo) * vo + tx(uci, vfi) * uo * (1 - vo) +
+               tx(uci, vci) * uo * vo;
+        return c * l.scale;
+    }
     if (dot(lp.n, view_dir) <= 0) return {0, 0, 0};
     return l.intensity;
 }

@@ -130,7 +130,7 @@ __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &
     if (alive && !scatter && v.area_light_id >= 0) {
         const LightRec &l = S.lights[v.area_light_id];
         LightPoint lp{v.pos, v.g.ng};
-        D3 Le = light_emission(l, -dir, lp);
+        D3 Le = light_emission(S, l, -dir, lp);
         if (never_scatter) {
             rad = rad + tp * Le;
         } else {
@@ -180,11 +180,12 @@ __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &
                     pdf_f = pv;
                     pre = pre * sigma_s_at(m, p_scatter);
                 } else {
-                    const MatRec &mat = S.materials[v.material_id];
+                    MatRec mat = S.materials[v.material_id];
+                    if (mat.textured) resolve_textures(S, mat, v.tu, v.tv, 0.0);
                     f = bsdf_eval<-1>(mat, dir_view, dir_light, v.g);
                     pdf_f = bsdf_pdf<-1>(mat, dir_view, dir_light, v.g);
                 }
-                D3 L = light_emission(l, -dir_light, lp);
+                D3 L = light_emission(S, l, -dir_light, lp);
                 D3 K = pre * G * f * L;
                 if (K.x != 0 || K.y != 0 || K.z != 0) {
                     push_sh = true;
@@ -212,7 +213,8 @@ __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &
                 P.ray_d[pid] = make_float4(float(nd.x), float(nd.y), float(nd.z), __builtin_huge_valf());
             }
         } else {
-            const MatRec &mat = S.materials[v.material_id];
+            MatRec mat = S.materials[v.material_id];
+            if (mat.textured) resolve_textures(S, mat, v.tu, v.tv, 0.0);
             double u0 = pcg_double(st, inc), u1 = pcg_double(st, inc), w = pcg_double(st, inc);
             BsdfSample bs = bsdf_sample<-1>(mat, dir_view, v.g, u0, u1, w);
             if (!bs.valid) {

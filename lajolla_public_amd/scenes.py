@@ -101,6 +101,7 @@ class SceneDesc:
         self.shapes = []
         self.lights = []
         self.media = []
+        self.textures = []
         self.envmap_light_id = -1
         self.options = dict(integrator=capi.INTEGRATOR_PATH, samples_per_pixel=4,
                             max_depth=-1, rr_depth=5, vol_path_version=0,
@@ -109,9 +110,14 @@ class SceneDesc:
         self.meta = {}
 
     # -- builders
-    def add_material(self, type, color_a=(0, 0, 0), color_b=(0, 0, 0), roughness=0.0, eta=1.5, **kw):
+    def add_material(self, type, color_a=(0, 0, 0), color_b=(0, 0, 0), roughness=0.0, eta=1.5, textures=None,
+                     **kw):
+        """textures: {field name: texture id from add_image_texture /
+        add_checkerboard} for the fields that vary over the surface."""
         m = dict(type=type, color_a=tuple(color_a), color_b=tuple(color_b),
-                 roughness=roughness, eta=eta)
+                 roughness=roughness, eta=eta, textures=dict(textures or {}))
+        for k in m["textures"]:
+            assert k in capi.TEXTURE_SLOTS, k
         for k in ("specular_transmission", "metallic", "subsurface", "specular", "specular_tint",
                   "anisotropic", "sheen", "sheen_tint", "clearcoat", "clearcoat_gloss"):
             m[k] = kw.get(k, 0.0)
@@ -143,6 +149,34 @@ class SceneDesc:
                                 intensity=tuple(intensity)))
         self.shapes[shape_id]["area_light_id"] = len(self.lights) - 1
         return len(self.lights) - 1
+
+    def add_image_texture(self, data, uscale=1.0, vscale=1.0, uoffset=0.0, voffset=0.0):
+        """ImageTexture (src/texture.h:82-88): data (h, w) or (h, w, 1|3)
+        floats, row 0 at the top."""
+        a = np.ascontiguousarray(data, np.float32)
+        if a.ndim == 2:
+            a = a[..., None]
+        assert a.ndim == 3 and a.shape[2] in (1, 3)
+        self.textures.append(dict(type=capi.TEXTURE_IMAGE, data=a, color0=(0, 0, 0), color1=(0, 0, 0),
+                                  uscale=uscale, vscale=vscale, uoffset=uoffset, voffset=voffset))
+        return len(self.textures) - 1
+
+    def add_checkerboard(self, color0=(0.4, 0.4, 0.4), color1=(0.2, 0.2, 0.2), uscale=1.0, vscale=1.0, uoffset=0.0,
+                         voffset=0.0):
+        """CheckerboardTexture (src/texture.h:90-97; defaults of parse_scene.cpp:349-350)."""
+        self.textures.append(dict(type=capi.TEXTURE_CHECKERBOARD, data=None, color0=tuple(color0),
+                                  color1=tuple(color1), uscale=uscale, vscale=vscale, uoffset=uoffset,
+                                  voffset=voffset))
+        return len(self.textures) - 1
+
+    def add_envmap(self, image, scale=1.0, to_world=None):
+        """An environment map of an (h, w, 3) lat-long image (src/lights/envmap.inl)."""
+        self.lights.append(dict(type=capi.LIGHT_ENVMAP, shape_id=-1, intensity=(0, 0, 0),
+                                env=np.ascontiguousarray(image, np.float32).reshape(image.shape[0], image.shape[1], 3),
+                                to_world=np.eye(4) if to_world is None else np.asarray(to_world),
+                                scale=scale))
+        self.envmap_light_id = len(self.lights) - 1
+        return self.envmap_light_id
 
     def add_constant_envmap(self, rgb, scale=1.0, to_world=None):
         self.lights.append(dict(type=capi.LIGHT_ENVMAP, shape_id=-1, intensity=(0, 0, 0),
@@ -206,6 +240,18 @@ class SceneDesc:
             for k in ("roughness", "eta", "specular_transmission", "metallic", "subsurface", "specular",
                       "specular_tint", "anisotropic", "sheen", "sheen_tint", "clearcoat", "clearcoat_gloss"):
                 setattr(o, k, float(m[k]))
+            for k, tid in m.get("textures", {}).items():
+                o.texture[capi.TEXTURE_SLOTS.index(k)] = tid + 1
+        texs = (capi.TextureDesc * max(1, len(self.textures)))()
+        for i, t in enumerate(self.textures):
+            o = texs[i]
+            o.type = t["type"]
+            o.color0 = capi.D3(*t["color0"])
+            o.color1 = capi.D3(*t["color1"])
+            o.uscale, o.vscale, o.uoffset, o.voffset = t["uscale"], t["vscale"], t["uoffset"], t["voffset"]
+            if t["type"] == capi.TEXTURE_IMAGE:
+                o.height, o.width, o.channels = t["data"].shape
+                o.data = ptr(t["data"], C.c_float)
         lights = (capi.LightDesc * max(1, len(self.lights)))()
         for i, l in enumerate(self.lights):
             o = lights[i]
@@ -241,7 +287,9 @@ class SceneDesc:
                     ov.p_min = capi.D3(*v["p_min"])
                     ov.p_max = capi.D3(*v["p_max"])
                     ov.data = ptr(data, C.c_float)
-        keep += [shapes, mats, lights, media]
+        keep += [shapes, mats, lights, media, texs]
+        d.textures = C.cast(texs, C.POINTER(capi.TextureDesc))
+        d.num_textures = len(self.textures)
         d.shapes = C.cast(shapes, C.POINTER(capi.ShapeDesc))
         d.num_shapes = len(self.shapes)
         d.materials = C.cast(mats, C.POINTER(capi.MaterialDesc))
@@ -702,4 +750,63 @@ def hetvol_smoke(seed=3, width=768, height=576, spp=64, max_depth=16, res=128, s
     s.options.update(integrator=capi.INTEGRATOR_VOLPATH, samples_per_pixel=spp, max_depth=max_depth,
                      rr_depth=5)
     s.meta = dict(seed=seed, res=res, scale=scale, colored=colored)
+    return s
+
+
+def sky_image(w=64, h=32, sun=(0.3, 0.25), seed=0):
+    """A lat-long sky: a blue-to-white gradient with a bright sun texel block
+    at (u, v) = sun, and a dark lower hemisphere."""
+    rng = np.random.default_rng(seed)
+    v = (np.arange(h) + 0.5) / h
+    img = np.zeros((h, w, 3), np.float32)
+    up = np.clip(1 - 2 * v, 0, 1)[:, None]
+    img[:] = (0.3 + 0.7 * up)[..., None] * np.array([0.4, 0.6, 1.0]) + (1 - up)[..., None] * 0.05
+    img *= 1 + 0.05 * rng.standard_normal((h, w, 1))
+    su, sv = int(sun[0] * w), int(sun[1] * h)
+    img[max(sv - 1, 0):sv + 1, max(su - 1, 0):su + 1] = [60.0, 50.0, 40.0]
+    return np.clip(img, 0, None).astype(np.float32)
+
+
+def textured_showcase(seed=5, width=96, height=72, spp=16, max_depth=6, envmap=True):
+    """Textured materials under an image environment map: a checkerboard
+    floor, spheres with image-textured colour, a RoughPlastic with a
+    roughness texture, a DisneyBSDF with metallic and base-colour textures,
+    a RoughDielectric with a coloured transmittance image, and a small area
+    light."""
+    rng = np.random.default_rng(seed)
+    s = SceneDesc()
+    s.name = "textured_showcase"
+    chk = s.add_checkerboard((0.8, 0.8, 0.8), (0.1, 0.12, 0.15), uscale=8, vscale=8)
+    img = rng.uniform(0.05, 0.95, (32, 32, 3)).astype(np.float32)
+    img_tex = s.add_image_texture(img, uscale=1 / (2 * np.pi), vscale=1 / np.pi)  # sphere uv is in radians
+    rough = s.add_image_texture(np.tile(np.linspace(0.05, 0.9, 16, dtype=np.float32), (16, 1)),
+                                uscale=1 / (2 * np.pi), vscale=1 / np.pi)
+    metal = s.add_checkerboard((1, 1, 1), (0, 0, 0), uscale=2 / np.pi, vscale=2 / np.pi)
+    trans = s.add_image_texture(rng.uniform(0.5, 1.0, (8, 8, 3)).astype(np.float32), uscale=1 / np.pi,
+                                vscale=1 / np.pi)
+    floor = s.add_material(capi.MAT_LAMBERTIAN, textures={"color_a": chk})
+    Pf = np.array([[-6, 0, -6], [6, 0, -6], [6, 0, 6], [-6, 0, 6]], np.float64)
+    s.add_mesh(Pf, np.array([[0, 2, 1], [0, 3, 2]]), floor, uvs=(Pf[:, [0, 2]] + 6) / 12,
+               normals=np.tile([0, 1, 0], (4, 1)))
+    m_img = s.add_material(capi.MAT_LAMBERTIAN, textures={"color_a": img_tex})
+    m_plastic = s.add_material(capi.MAT_ROUGH_PLASTIC, (0.6, 0.2, 0.1), (1, 1, 1), roughness=0.3, eta=1.5,
+                               textures={"roughness": rough})
+    m_disney = s.add_material(capi.MAT_DISNEY_BSDF, (0.9, 0.6, 0.3), roughness=0.35, eta=1.5, specular=0.5,
+                              textures={"metallic": metal, "color_a": img_tex})
+    m_glass = s.add_material(capi.MAT_ROUGH_DIELECTRIC, (1, 1, 1), (1, 1, 1), roughness=0.05, eta=1.5,
+                             textures={"color_b": trans})
+    # the spheres float 5 cm above the floor: a tangent contact would leave which surface a ray
+    # meets first to rounding
+    for mid, c in ((m_img, (-1.8, 0.75, 0.0)), (m_plastic, (-0.6, 0.75, 0.8)), (m_disney, (0.6, 0.75, -0.6)),
+                   (m_glass, (1.8, 0.75, 0.4))):
+        s.add_sphere(c, 0.7, mid)
+    black = s.add_material(capi.MAT_LAMBERTIAN, (0, 0, 0))
+    Pl, Il, Nl = quad((-0.5, 4, -0.5), (0.5, 4, -0.5), (0.5, 4, 0.5), (-0.5, 4, 0.5))
+    lid = s.add_mesh(Pl, Il, black, normals=Nl)
+    s.add_area_light(lid, (8.0, 8.0, 8.0))
+    if envmap:
+        s.add_envmap(sky_image(seed=seed), scale=1.0)
+    s.camera.update(cam_to_world=look_at((0.5, 2.2, 5.5), (0, 0.5, 0), (0, 1, 0)), fov=50.0, width=width,
+                    height=height)
+    s.options.update(integrator=capi.INTEGRATOR_PATH, samples_per_pixel=spp, max_depth=max_depth, rr_depth=5)
     return s

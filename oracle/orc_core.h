@@ -398,13 +398,15 @@ inline V3 sample_cos_hemisphere(V2 r) {
 }
 
 // ------------------------------------------------------------- materials
-struct Material {  // material.h:12-98, constant textures only
+struct Material {  // material.h:12-98
     int type = 0;
     V3 a, b;  // see ljx_material_desc
     Real roughness = 0, eta = 1;
     Real specular_transmission = 0, metallic = 0, subsurface = 0, specular = 0,
          specular_tint = 0, anisotropic = 0, sheen = 0, sheen_tint = 0, clearcoat = 0,
          clearcoat_gloss = 0;
+    int tex[13] = {0};  // per field: 0 the constant above, k > 0 the scene's texture k - 1
+    bool textured = false;
 };
 struct ShadePoint {  // the PathVertex fields a BSDF reads (intersection.h:15-35)
     V3 geometric_normal;

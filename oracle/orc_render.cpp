@@ -37,11 +37,99 @@ struct Shape {  // shape.h:20-51
     Real total_area = 0;
     TableDist1D triangle_sampler;
 };
+// ImageTexture / CheckerboardTexture (texture.h:82-97) with the image's
+// mipmap (mipmap.h:7-48).  1-channel images hold their value in every channel.
+struct Texture {
+    int type = LJX_TEXTURE_IMAGE;
+    int channels = 3;
+    std::vector<std::vector<V3>> levels;
+    std::vector<int> w, h;
+    V3 color0, color1;
+    Real uscale = 1, vscale = 1, uoffset = 0, voffset = 0;
+};
+// mipmap.h:25-48: 2x2 box filter, at most 8 levels.  Where a level's width or
+// height is already 1 the reference reads the pixel past the row or column
+// (outside the image on its last row); here that read is clamped to the image.
+static void make_mipmap(Texture &t, const float *data, int w, int h, int channels) {
+    std::vector<V3> img(size_t(w) * h);
+    for (size_t i = 0; i < img.size(); i++)
+        img[i] = channels == 1 ? V3{data[i], data[i], data[i]}
+                               : V3{data[3 * i], data[3 * i + 1], data[3 * i + 2]};
+    t.levels = {img};
+    t.w = {w};
+    t.h = {h};
+    int size = std::max(w, h);
+    int num_levels = std::min(int(std::ceil(std::log2(Real(size)) + 1)), 8);
+    for (int i = 1; i < num_levels; i++) {
+        const std::vector<V3> &prev = t.levels.back();
+        int pw = t.w.back(), ph = t.h.back();
+        int nw = std::max(pw / 2, 1), nh = std::max(ph / 2, 1);
+        std::vector<V3> next(size_t(nw) * nh);
+        auto at = [&](int x, int y) { return prev[size_t(std::min(y, ph - 1)) * pw + std::min(x, pw - 1)]; };
+        for (int y = 0; y < nh; y++)
+            for (int x = 0; x < nw; x++)
+                next[size_t(y) * nw + x] =
+                    (at(2 * x, 2 * y) + at(2 * x + 1, 2 * y) + at(2 * x, 2 * y + 1) + at(2 * x + 1, 2 * y + 1)) / Real(4);
+        t.levels.push_back(next);
+        t.w.push_back(nw);
+        t.h.push_back(nh);
+    }
+}
+// lajolla.h:48-61
+static int imod(int a, int b) {
+    int r = a % b;
+    return r < 0 ? r + b : r;
+}
+static Real rmod(Real a, Real b) {
+    Real r = std::fmod(a, b);
+    return r < 0 ? r + b : r;
+}
+// mipmap.h:50-72: bilinear on one level, pixel centres at half-integers
+static V3 mip_lookup(const Texture &t, Real u, Real v, int level) {
+    const int w = t.w[level], h = t.h[level];
+    u = u * w - Real(0.5);
+    v = v * h - Real(0.5);
+    int ufi = imod(int(u), w), vfi = imod(int(v), h);
+    int uci = imod(ufi + 1, w), vci = imod(vfi + 1, h);
+    Real uo = u - ufi, vo = v - vfi;
+    const std::vector<V3> &img = t.levels[level];
+    V3 ff = img[size_t(vfi) * w + ufi], fc = img[size_t(vci) * w + ufi];
+    V3 cf = img[size_t(vfi) * w + uci], cc = img[size_t(vci) * w + uci];
+    return ff * (1 - uo) * (1 - vo) + fc * (1 - uo) * vo + cf * uo * (1 - vo) + cc * uo * vo;
+}
+// mipmap.h:74-88: trilinear between levels
+static V3 mip_lookup(const Texture &t, Real u, Real v, Real level) {
+    const int n = int(t.levels.size());
+    if (level <= 0) return mip_lookup(t, u, v, 0);
+    if (level < Real(n - 1)) {
+        int fl = std::clamp(int(std::floor(level)), 0, n - 1), cl = std::clamp(fl + 1, 0, n - 1);
+        Real lo = level - fl;
+        return mip_lookup(t, u, v, fl) * (1 - lo) + mip_lookup(t, u, v, cl) * lo;
+    }
+    return mip_lookup(t, u, v, n - 1);
+}
+// texture.h:131-154
+static V3 tex_eval(const Texture &t, V2 uv, Real footprint) {
+    Real lu = rmod(uv.x * t.uscale + t.uoffset, Real(1)), lv = rmod(uv.y * t.vscale + t.voffset, Real(1));
+    if (t.type == LJX_TEXTURE_CHECKERBOARD) {
+        int x = 2 * imod(int(lu * 2), 2) - 1, y = 2 * imod(int(lv * 2), 2) - 1;
+        return x * y == 1 ? t.color0 : t.color1;
+    }
+    Real scaled = std::max(t.w[0], t.h[0]) * std::max(t.uscale, t.vscale) * footprint;
+    Real level = std::log2(std::max(scaled, Real(1e-8f)));
+    return mip_lookup(t, lu, lv, level);
+}
+// a Texture<Real> field: a 1-channel image as is, otherwise the average
+// (parse_scene.cpp:480-487 for the checkerboard)
+static Real tex_eval1(const Texture &t, V2 uv, Real footprint) {
+    V3 c = tex_eval(t, uv, footprint);
+    return (t.type == LJX_TEXTURE_IMAGE && t.channels == 1) ? c.x : avg(c);
+}
 struct Light {  // light.h:12-30
     int type = 0;
     int shape_id = -1;
     V3 intensity;
-    V3 env_value;  // 1x1 image texel
+    Texture env;  // the envmap's image (uv scale 1, offset 0)
     M4 to_world, to_local;
     Real scale = 1;
     TableDist2D dist;
@@ -75,6 +163,7 @@ struct PathVertex {  // intersection.h:15-35
     Frame shading_frame;
     V2 st, uv;
     Real mean_curvature = 0;
+    Real inv_uv_size = 1, uv_screen_size = 0;  // intersection.h:25-30
     int shape_id = -1, primitive_id = -1, material_id = -1;
     int interior_medium_id = -1, exterior_medium_id = -1;
 };
@@ -92,6 +181,7 @@ struct Scene {
     std::vector<Shape> shapes;
     std::vector<Light> lights;
     std::vector<Medium> media;
+    std::vector<Texture> textures;
     int envmap_light_id = -1;
     Real bound_radius = 0;
     V3 bound_center;
@@ -429,10 +519,12 @@ static void shading_info(const Shape &sh, PathVertex &v) {
     if (sh.type == LJX_SHAPE_SPHERE) {
         Real r = sh.radius;
         V3 dpdu{-r * std::sin(v.st.x) * std::sin(v.st.y), r * std::cos(v.st.x) * std::sin(v.st.y), 0};
+        V3 dpdv{r * std::cos(v.st.x) * std::cos(v.st.y), r * std::sin(v.st.x) * std::cos(v.st.y), -r * std::sin(v.st.y)};
         V3 tangent = normalize(dpdu - v.geometric_normal * dot(v.geometric_normal, dpdu));
         v.shading_frame = Frame(tangent, normalize(cross(v.geometric_normal, tangent)), v.geometric_normal);
         v.uv = v.st;
         v.mean_curvature = 1 / r;
+        v.inv_uv_size = (length(dpdu) + length(dpdv)) / 2;
         return;
     }
     int i0 = sh.indices[3 * v.primitive_id], i1 = sh.indices[3 * v.primitive_id + 1],
@@ -456,13 +548,15 @@ static void shading_info(const Shape &sh, PathVertex &v) {
     Real det = duvds.x * duvdt.y - duvdt.x * duvds.y;
     Real dsdu = duvdt.y / det, dtdu = -duvds.y / det;
     Real dsdv = duvdt.x / det, dtdv = -duvds.x / det;
-    V3 dpdu;
+    V3 dpdu, dpdv;
     if (std::fabs(det) > 1e-8f) {
         V3 dpds = p2 - p0, dpdt = p2 - p1;
         dpdu = dpds * dsdu + dpdt * dtdu;
+        dpdv = dpds * dsdv + dpdt * dtdv;
     } else {
-        dpdu = coordinate_system(v.geometric_normal).first;
+        std::tie(dpdu, dpdv) = coordinate_system(v.geometric_normal);
     }
+    v.inv_uv_size = std::max(length(dpdu), length(dpdv));
     V3 n = v.geometric_normal;
     if (!sh.normals.empty()) {
         n = normalize((1 - s0 - s1) * sh.normals[i0] + s0 * sh.normals[i1] + s1 * sh.normals[i2]);
@@ -478,8 +572,10 @@ static void shading_info(const Shape &sh, PathVertex &v) {
     }
 }
 
-// intersection.cpp:7-65
-static bool intersect(const Scene &s, const Ray &ray, PathVertex &v, uint64_t *count) {
+// intersection.cpp:7-65; spread is the ray differential's (its radius is 0
+// wherever the reference passes one), so the footprint ray_radius / inv_uv_size
+// is zero except on camera rays (path_tracing.h:14-16)
+static bool intersect(const Scene &s, const Ray &ray, PathVertex &v, uint64_t *count, Real spread = 0) {
     if (count) (*count)++;
     Ray r = to_f32(ray);
     HitRec h;
@@ -497,6 +593,7 @@ static bool intersect(const Scene &s, const Ray &ray, PathVertex &v, uint64_t *c
     v.st = V2{Real(float(h.u)), Real(float(h.v))};
     shading_info(sh, v);
     if (dot(v.geometric_normal, v.shading_frame.n) < 0) v.geometric_normal = -v.geometric_normal;
+    v.uv_screen_size = (0 + spread * distance(ray.org, v.position)) / v.inv_uv_size;
     return true;
 }
 // intersection.cpp:67-85
@@ -566,7 +663,7 @@ static Real surface_area(const Shape &sh) {
     if (sh.type == LJX_SHAPE_SPHERE) return 4 * kPi * sh.radius * sh.radius;
     return sh.total_area;
 }
-// envmap.inl:1-98 for a 1x1 image texture.
+// envmap.inl:1-98
 static V2 env_uv(const Light &l, V3 world_dir) {
     V3 ld = xform_vector(l.to_local, world_dir);
     V2 uv{std::atan2(ld.x, -ld.z) * kInvTwoPi, std::acos(std::clamp(ld.y, Real(-1), Real(1))) * kInvPi};
@@ -591,13 +688,21 @@ static Real pdf_point_on_light(const Scene &s, const Light &l, const PointAndNor
     if (se <= 0) return 0;
     return pdf(l.dist, uv) / (2 * kPi * kPi * se);
 }
-// diffuse_area_light.inl:15-20; envmap.inl:44-72 (1x1: the texel value)
+// diffuse_area_light.inl:15-20; envmap.inl:44-72.  The envmap's footprint is
+// min(|du/dw|, dv/dw) as the reference forms it; dv/dw is never positive, so
+// the image is read on level 0.
 static V3 light_emission(const Light &l, V3 view_dir, const PointAndNormal &pn) {
     if (l.type == LJX_LIGHT_DIFFUSE_AREA) {
         if (dot(pn.normal, view_dir) <= 0) return {};
         return l.intensity;
     }
-    return l.env_value * l.scale;
+    V3 w = xform_vector(l.to_local, -view_dir);
+    V2 uv{std::atan2(w.x, -w.z) * kInvTwoPi, std::acos(std::clamp(w.y, Real(-1), Real(1))) * kInvPi};
+    if (uv.x < 0) uv.x += 1;
+    Real dudwx = -w.z / (w.x * w.x + w.z * w.z), dudwz = w.x / (w.x * w.x + w.z * w.z);
+    Real dvdwy = -1 / std::sqrt(std::max(1 - w.y * w.y, Real(0)));
+    Real footprint = std::min(std::sqrt(dudwx * dudwx + dudwz * dudwz), dvdwy);
+    return tex_eval(l.env, uv, footprint) * l.scale;
 }
 static Real light_power(const Scene &s, const Light &l) {
     if (l.type == LJX_LIGHT_DIFFUSE_AREA) return luminance(l.intensity) * surface_area(s.shapes[l.shape_id]) * kPi;
@@ -610,6 +715,34 @@ static V3 vertex_emission(const Scene &s, const PathVertex &v, V3 view_dir) {
 
 static ShadePoint shade_point(const PathVertex &v) { return ShadePoint{v.geometric_normal, v.shading_frame}; }
 
+// The material with its texture fields evaluated at the vertex's uv and
+// footprint, as each BSDF's eval(texture, vertex.uv, vertex.uv_screen_size)
+// calls do (e.g. lambertian.inl:11-13, roughplastic.inl:30-36).
+static Material resolve(const Scene &s, const Material &m, const PathVertex &v) {
+    if (!m.textured) return m;
+    Material r = m;
+    auto spec = [&](int k, V3 &dst) {
+        if (m.tex[k]) dst = tex_eval(s.textures[m.tex[k] - 1], v.uv, v.uv_screen_size);
+    };
+    auto real = [&](int k, Real &dst) {
+        if (m.tex[k]) dst = tex_eval1(s.textures[m.tex[k] - 1], v.uv, v.uv_screen_size);
+    };
+    spec(0, r.a);
+    spec(1, r.b);
+    real(2, r.roughness);
+    real(3, r.specular_transmission);
+    real(4, r.metallic);
+    real(5, r.subsurface);
+    real(6, r.specular);
+    real(7, r.specular_tint);
+    real(8, r.anisotropic);
+    real(9, r.sheen);
+    real(10, r.sheen_tint);
+    real(11, r.clearcoat);
+    real(12, r.clearcoat_gloss);
+    return r;
+}
+
 struct Counters {
     uint64_t ext = 0, shadow = 0;
 };
@@ -621,7 +754,8 @@ static V3 path_tracing(const Scene &s, int x, int y, Pcg &rng, Counters &cnt) {
     V2 screen{(x + pcg_double(rng)) / w, (y + pcg_double(rng)) / h};
     Ray ray = sample_primary(s.camera, screen);
     PathVertex vertex;
-    if (!intersect(s, ray, vertex, &cnt.ext)) {
+    // init_ray_differential (ray.h:35-37): radius 0, spread 0.25 / max(w, h)
+    if (!intersect(s, ray, vertex, &cnt.ext, Real(0.25) / std::max(w, h))) {
         if (s.envmap_light_id >= 0) {
             const Light &env = s.lights[s.envmap_light_id];
             return light_emission(env, -ray.dir, PointAndNormal{});
@@ -635,7 +769,7 @@ static V3 path_tracing(const Scene &s, int x, int y, Pcg &rng, Counters &cnt) {
     if (s.shapes[vertex.shape_id].area_light_id >= 0) radiance += throughput * vertex_emission(s, vertex, -ray.dir);
     int max_depth = s.options.max_depth;
     for (int nv = 3; max_depth == -1 || nv <= max_depth + 1; nv++) {
-        const Material &mat = s.materials[vertex.material_id];
+        const Material mat = resolve(s, s.materials[vertex.material_id], vertex);
         ShadePoint sp = shade_point(vertex);
         V2 light_uv{pcg_double(rng), pcg_double(rng)};
         Real light_w = pcg_double(rng);
@@ -975,7 +1109,7 @@ static V3 vol_path_tracing(const Scene &s, int x, int y, Pcg &rng, bool nee_stre
             multi_trans_pdf = V3{1, 1, 1};
             ray = Ray{p, *nd, 0, inf()};
         } else {
-            const Material &mat = s.materials[v.material_id];
+            const Material mat = resolve(s, s.materials[v.material_id], v);
             ShadePoint sp = shade_point(v);
             V3 C = vol_nee(s, v.position, dir_view, medium, bounces, *nr, &mat, &sp, nullptr, cnt);
             radiance += throughput * C;
@@ -1044,7 +1178,39 @@ static int make_scene(const ljx_scene_desc *d, std::unique_ptr<Scene> &out) {
         q.sheen_tint = m.sheen_tint;
         q.clearcoat = m.clearcoat;
         q.clearcoat_gloss = m.clearcoat_gloss;
+        for (int k = 0; k < LJX_MAT_TEXTURE_SLOTS; k++) {
+            if (m.texture[k] < 0 || m.texture[k] > d->num_textures) {
+                g_err = "material texture index out of range";
+                return LJX_ERR_INVALID_ARGUMENT;
+            }
+            q.tex[k] = m.texture[k];
+            q.textured = q.textured || m.texture[k] != 0;
+        }
         s->materials.push_back(q);
+    }
+    if (d->num_textures < 0 || (d->num_textures > 0 && !d->textures)) {
+        g_err = "bad texture array";
+        return LJX_ERR_INVALID_ARGUMENT;
+    }
+    for (int i = 0; i < d->num_textures; i++) {
+        const ljx_texture_desc &t = d->textures[i];
+        Texture q;
+        q.type = int(t.type);
+        q.uscale = t.uscale, q.vscale = t.vscale, q.uoffset = t.uoffset, q.voffset = t.voffset;
+        q.color0 = v3(t.color0);
+        q.color1 = v3(t.color1);
+        if (q.type == LJX_TEXTURE_IMAGE) {
+            if (!t.data || t.width < 1 || t.height < 1 || (t.channels != 1 && t.channels != 3)) {
+                g_err = "image texture: data, a positive size and 1 or 3 channels are required";
+                return LJX_ERR_INVALID_ARGUMENT;
+            }
+            q.channels = t.channels;
+            make_mipmap(q, t.data, t.width, t.height, t.channels);
+        } else if (q.type != LJX_TEXTURE_CHECKERBOARD) {
+            g_err = "unknown texture type";
+            return LJX_ERR_INVALID_ARGUMENT;
+        }
+        s->textures.push_back(std::move(q));
     }
     for (int i = 0; i < d->num_media; i++) {
         const ljx_medium_desc &m = d->media[i];
@@ -1145,17 +1311,27 @@ static int make_scene(const ljx_scene_desc *d, std::unique_ptr<Scene> &out) {
         q.shape_id = l.shape_id;
         q.intensity = v3(l.intensity);
         if (q.type == LJX_LIGHT_ENVMAP) {
-            if (l.env_width != 1 || l.env_height != 1 || !l.env_rgb) {
-                g_err = "only 1x1 environment maps are supported";
-                return LJX_ERR_UNSUPPORTED;
+            if (l.env_width < 1 || l.env_height < 1 || !l.env_rgb) {
+                g_err = "environment map: texels and a positive size are required";
+                return LJX_ERR_INVALID_ARGUMENT;
             }
-            q.env_value = V3{l.env_rgb[0], l.env_rgb[1], l.env_rgb[2]};
+            const int w = l.env_width, h = l.env_height;
+            make_mipmap(q.env, l.env_rgb, w, h, 3);
             q.to_world = m4_from(l.to_world);
             q.to_local = inverse(q.to_world);
             q.scale = l.scale;
-            // envmap.inl:75-98: luminance * sin(elevation) at texel centres
-            std::vector<Real> f{luminance(q.env_value) * std::sin(kPi * Real(0.5))};
-            q.dist = make_table_dist_2d(f, 1, 1);
+            // envmap.inl:75-98: luminance * sin(elevation) at texel centres of level 0
+            std::vector<Real> f(size_t(w) * h);
+            size_t k = 0;
+            for (int y = 0; y < h; y++) {
+                Real v = (y + Real(0.5)) / Real(h);
+                Real se = std::sin(kPi * v);
+                for (int x = 0; x < w; x++) {
+                    Real u = (x + Real(0.5)) / Real(w);
+                    f[k++] = luminance(mip_lookup(q.env, u, v, 0)) * se;
+                }
+            }
+            q.dist = make_table_dist_2d(f, w, h);
         }
         s->lights.push_back(q);
     }
@@ -1387,7 +1563,7 @@ int orc_render(void *sp, const ljx_render_options *opts, int plan, int threads, 
                 Ray ray = sample_primary(s->camera, V2{(x + 0.5) / w, (y + 0.5) / h});
                 PathVertex v;
                 V3 c;
-                if (intersect(*s, ray, v, nullptr)) {
+                if (intersect(*s, ray, v, nullptr, Real(0.25) / std::max(w, h))) {
                     Real dist = length(v.position - ray.org);
                     if (mode == LJX_INTEGRATOR_DEPTH) c = V3{dist, dist, dist};
                     else if (mode == LJX_INTEGRATOR_SHADING_NORMAL) c = v.shading_frame.n;
@@ -1395,7 +1571,18 @@ int orc_render(void *sp, const ljx_render_options *opts, int plan, int threads, 
                         c = V3{v.mean_curvature, v.mean_curvature, v.mean_curvature};
                     else if (mode == LJX_INTEGRATOR_RAY_DIFFERENTIAL)
                         c = V3{0, Real(0.25) / std::max(w, h), 0};  // the camera's differential (ray.h:35-37)
-                    // LJX_INTEGRATOR_MIPMAP_LEVEL: constant textures have no level: black
+                    else if (mode == LJX_INTEGRATOR_MIPMAP_LEVEL && v.material_id >= 0) {
+                        // the level the footprint selects in get_texture(material), an image texture
+                        // (material.cpp:121-123: the first colour field; DisneyClearcoat has none)
+                        const Material &m = s->materials[v.material_id];
+                        if (m.type != LJX_MAT_DISNEY_CLEARCOAT && m.tex[0] &&
+                            s->textures[m.tex[0] - 1].type == LJX_TEXTURE_IMAGE) {
+                            const Texture &t = s->textures[m.tex[0] - 1];
+                            Real sf = std::max(t.w[0], t.h[0]) * std::max(t.uscale, t.vscale) * v.uv_screen_size;
+                            Real level = std::log2(std::max(sf, Real(1e-8f)));
+                            c = V3{level, level, level};
+                        }
+                    }
                 }
                 size_t o = (size_t(y) * w + x) * 3;
                 rgb_out[o] = float(c.x);
@@ -1538,6 +1725,49 @@ int orc_filter_sample(uint32_t type, double param, const double *rnd, size_t n, 
     return LJX_OK;
 }
 // Phase function eval/pdf/sample; dirs: n*3 in, n*3 out, rnd: n*2.
+// A texture of an ljx_texture_desc evaluated at n (u, v, footprint) triples
+// (texture.h:131-154): out[3 i..3 i+2] for a spectrum field (scalar = 0), or
+// out[i] for a scalar field (scalar = 1).
+int orc_texture_eval(const ljx_texture_desc *d, const double *q, size_t n, int scalar, double *out) {
+    Texture t;
+    t.type = int(d->type);
+    t.uscale = d->uscale, t.vscale = d->vscale, t.uoffset = d->uoffset, t.voffset = d->voffset;
+    t.color0 = v3(d->color0);
+    t.color1 = v3(d->color1);
+    if (t.type == LJX_TEXTURE_IMAGE) {
+        if (!d->data || d->width < 1 || d->height < 1 || (d->channels != 1 && d->channels != 3))
+            return LJX_ERR_INVALID_ARGUMENT;
+        t.channels = d->channels;
+        make_mipmap(t, d->data, d->width, d->height, d->channels);
+    }
+    for (size_t i = 0; i < n; i++) {
+        V2 uv{q[3 * i], q[3 * i + 1]};
+        if (scalar) {
+            out[i] = tex_eval1(t, uv, q[3 * i + 2]);
+        } else {
+            V3 c = tex_eval(t, uv, q[3 * i + 2]);
+            out[3 * i] = c.x, out[3 * i + 1] = c.y, out[3 * i + 2] = c.z;
+        }
+    }
+    return LJX_OK;
+}
+
+// The mipmap of an image texture (mipmap.h:25-48): levels written one after
+// another as RGB doubles; returns the level count, or a negative status.
+int orc_texture_levels(const ljx_texture_desc *d, double *out, size_t cap) {
+    if (!d->data || d->width < 1 || d->height < 1 || (d->channels != 1 && d->channels != 3))
+        return LJX_ERR_INVALID_ARGUMENT;
+    Texture t;
+    make_mipmap(t, d->data, d->width, d->height, d->channels);
+    size_t k = 0;
+    for (const auto &lv : t.levels)
+        for (const V3 &c : lv) {
+            if (k + 3 > cap) return LJX_ERR_INVALID_ARGUMENT;
+            out[k++] = c.x, out[k++] = c.y, out[k++] = c.z;
+        }
+    return int(t.levels.size());
+}
+
 int orc_phase_batch(uint32_t type, double g, const double *din, const double *dout, const double *rnd, size_t n,
                     double *val, double *pdfv, double *smp) {
     Phase p{int(type), g};
@@ -1556,6 +1786,7 @@ int orc_phase_batch(uint32_t type, double g, const double *din, const double *do
 int orc_light_sample(void *sp, int light_id, const double *ref, const double *rnd, size_t n, double *out_pn,
                      double *pdf_out) {
     Scene *s = static_cast<Scene *>(sp);
+    if (light_id < 0 || light_id >= int(s->lights.size())) return LJX_ERR_INVALID_ARGUMENT;
     const Light &l = s->lights[light_id];
     for (size_t i = 0; i < n; i++) {
         V3 r = v3(ref + 3 * i);
@@ -1567,6 +1798,21 @@ int orc_light_sample(void *sp, int light_id, const double *ref, const double *rn
         out_pn[6 * i + 4] = pn.normal.y;
         out_pn[6 * i + 5] = pn.normal.z;
         pdf_out[i] = pdf_point_on_light(*s, l, pn, r);
+    }
+    return LJX_OK;
+}
+// The environment map seen along n world directions (pointing away from the
+// scene): its radiance (envmap.inl:44-72) and the solid-angle pdf of sampling
+// that direction (envmap.inl:22-42).
+int orc_env_eval(void *sp, const double *dirs, size_t n, double *rgb_out, double *pdf_out) {
+    Scene *s = static_cast<Scene *>(sp);
+    if (s->envmap_light_id < 0) return LJX_ERR_INVALID_ARGUMENT;
+    const Light &l = s->lights[s->envmap_light_id];
+    for (size_t i = 0; i < n; i++) {
+        V3 d = v3(dirs + 3 * i);
+        V3 c = light_emission(l, -d, PointAndNormal{});
+        rgb_out[3 * i] = c.x, rgb_out[3 * i + 1] = c.y, rgb_out[3 * i + 2] = c.z;
+        pdf_out[i] = pdf_point_on_light(*s, l, PointAndNormal{V3{0, 0, 0}, -d}, V3{0, 0, 0});
     }
     return LJX_OK;
 }

@@ -49,8 +49,11 @@ def lib():
         L.orc_camera_rays.argtypes = [C.POINTER(capi.CameraDesc), dp, C.c_size_t, dp]
         L.orc_filter_sample.argtypes = [C.c_uint32, C.c_double, dp, C.c_size_t, dp]
         L.orc_phase_batch.argtypes = [C.c_uint32, C.c_double, dp, dp, dp, C.c_size_t, dp, dp, dp]
+        L.orc_texture_eval.argtypes = [C.POINTER(capi.TextureDesc), dp, C.c_size_t, C.c_int, dp]
+        L.orc_texture_levels.argtypes = [C.POINTER(capi.TextureDesc), dp, C.c_size_t]
         L.orc_light_sample.argtypes = [VP, C.c_int, dp, dp, C.c_size_t, dp, dp]
         L.orc_light_pmf.argtypes = [VP, dp]
+        L.orc_env_eval.argtypes = [VP, dp, C.c_size_t, dp, dp]
         L.orc_last_error.argtypes = [C.c_char_p, C.c_size_t]
         fp = C.POINTER(C.c_float)
         L.orc_guide_forward.argtypes = [dp, fp, C.c_int, C.c_int, dp]
@@ -158,6 +161,14 @@ class OracleScene:
                                        threads))
         return out
 
+    def env_eval(self, dirs):
+        """Envmap radiance and sampling pdf along world directions (n, 3)."""
+        dirs = np.ascontiguousarray(dirs, np.float64)
+        rgb = np.zeros((len(dirs), 3))
+        pdf = np.zeros(len(dirs))
+        _check(lib().orc_env_eval(self.h, _dp(dirs), len(dirs), _dp(rgb), _dp(pdf)))
+        return rgb, pdf
+
     def light_sample(self, light_id, refs, rnd):
         refs = np.ascontiguousarray(refs, np.float64)
         rnd = np.ascontiguousarray(rnd, np.float64)
@@ -225,6 +236,50 @@ def filter_sample(ftype, param, rnd):
     out = np.zeros(2 * n)
     lib().orc_filter_sample(ftype, param, _dp(rnd), n, _dp(out))
     return out
+
+
+def texture_desc(tex):
+    """An ljx_texture_desc for a texture dict of scenes.SceneDesc.textures
+    (the returned keep-alive list must outlive the call)."""
+    d = capi.TextureDesc()
+    d.type = tex["type"]
+    d.color0 = capi.D3(*tex["color0"])
+    d.color1 = capi.D3(*tex["color1"])
+    d.uscale, d.vscale, d.uoffset, d.voffset = tex["uscale"], tex["vscale"], tex["uoffset"], tex["voffset"]
+    keep = []
+    if tex["type"] == capi.TEXTURE_IMAGE:
+        a = np.ascontiguousarray(tex["data"], np.float32)
+        keep.append(a)
+        d.height, d.width, d.channels = a.shape
+        d.data = a.ctypes.data_as(C.POINTER(C.c_float))
+    return d, keep
+
+
+def texture_eval(tex, uv, footprint, scalar=False):
+    """Oracle texture lookups (texture.h:131-154) at uv (n, 2) with footprints (n,)."""
+    d, keep = texture_desc(tex)
+    q = np.ascontiguousarray(np.column_stack([np.asarray(uv, np.float64).reshape(-1, 2),
+                                              np.broadcast_to(np.asarray(footprint, np.float64), (len(uv),))]))
+    out = np.zeros(len(q) if scalar else 3 * len(q))
+    _check(lib().orc_texture_eval(C.byref(d), _dp(q), len(q), int(scalar), _dp(out)))
+    return out if scalar else out.reshape(-1, 3)
+
+
+def texture_levels(tex):
+    """The oracle's mipmap of an image texture: a list of (h, w, 3) arrays."""
+    d, keep = texture_desc(tex)
+    h, w = d.height, d.width
+    cap = 3 * 2 * w * h + 3 * 64
+    out = np.zeros(cap)
+    n = lib().orc_texture_levels(C.byref(d), _dp(out), cap)
+    if n < 0:
+        raise RuntimeError("orc_texture_levels failed")
+    levels, k = [], 0
+    for _ in range(n):
+        levels.append(out[k:k + 3 * w * h].reshape(h, w, 3))
+        k += 3 * w * h
+        w, h = max(w // 2, 1), max(h // 2, 1)
+    return levels
 
 
 def phase_batch(ptype, g, din, dout, rnd):

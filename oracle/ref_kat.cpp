@@ -14,6 +14,7 @@
 #include "pcg.h"
 #include "phase_function.h"
 #include "table_dist.h"
+#include "texture.h"
 #include "transform.h"
 
 #include <cstdio>
@@ -285,6 +286,98 @@ static void kat_phase() {
     std::printf("]}\n");
 }
 
+// Mipmaps and texture lookups (mipmap.h, texture.h), on power-of-two square
+// images: on other sizes make_mipmap reads outside the image at its last
+// levels, so there is no known answer to record.
+static void kat_texture() {
+    std::printf("{\"images\":[");
+    TexturePool pool;
+    const int sizes[3] = {8, 16, 4};
+    const int chans[3] = {3, 1, 3};
+    for (int k = 0; k < 3; k++) {
+        int n = sizes[k];
+        std::printf("%s{\"size\":%d,\"channels\":%d,\"data\":[", k ? "," : "", n, chans[k]);
+        if (chans[k] == 3) {
+            Image3 img(n, n);
+            for (int i = 0; i < n * n; i++) {
+                img(i) = Vector3{float(2 * U()), float(2 * U()), float(2 * U())};
+                std::printf("%s", i ? "," : "");
+                jd(img(i).x); std::printf(","); jd(img(i).y); std::printf(","); jd(img(i).z);
+            }
+            insert_image3(pool, "t" + std::to_string(k), img);
+        } else {
+            Image1 img(n, n);
+            for (int i = 0; i < n * n; i++) {
+                img(i) = float(2 * U());
+                std::printf("%s", i ? "," : "");
+                jd(img(i));
+            }
+            insert_image1(pool, "t" + std::to_string(k), img);
+        }
+        std::printf("],\"lookups\":[");
+        for (int q = 0; q < 64; q++) {
+            double us = q % 4 == 0 ? 1.0 : 0.25 + 3 * U(), vs = q % 4 == 0 ? 1.0 : 0.25 + 3 * U();
+            double uo = q % 3 == 0 ? 0.0 : 4 * U() - 2, vo = q % 3 == 0 ? 0.0 : 4 * U() - 2;
+            Vector2 uv{6 * U() - 3, 6 * U() - 3};
+            double fp = q % 8 == 0 ? 0.0 : pow(10.0, -4 + 4 * U());
+            Vector3 out;
+            if (chans[k] == 3) {
+                ImageTexture<Spectrum> t{int(pool.image3s.size()) - 1, us, vs, uo, vo};
+                out = eval(Texture<Spectrum>{t}, uv, fp, pool);
+            } else {
+                ImageTexture<Real> t{int(pool.image1s.size()) - 1, us, vs, uo, vo};
+                Real r = eval(Texture<Real>{t}, uv, fp, pool);
+                out = Vector3{r, r, r};
+            }
+            std::printf("%s{\"scale\":[", q ? "," : "");
+            jd(us); std::printf(","); jd(vs); std::printf("],\"offset\":["); jd(uo); std::printf(","); jd(vo);
+            std::printf("],\"uv\":["); jd(uv.x); std::printf(","); jd(uv.y);
+            std::printf("],\"footprint\":"); jd(fp); std::printf(",\"out\":"); jv(out); std::printf("}");
+        }
+        // every level of the mipmap
+        std::printf("],\"levels\":[");
+        if (chans[k] == 3) {
+            const Mipmap3 &m = pool.image3s.back();
+            for (size_t l = 0; l < m.images.size(); l++) {
+                std::printf("%s[", l ? "," : "");
+                for (int i = 0; i < m.images[l].width * m.images[l].height; i++) {
+                    if (i) std::printf(",");
+                    jv(m.images[l](i));
+                }
+                std::printf("]");
+            }
+        } else {
+            const Mipmap1 &m = pool.image1s.back();
+            for (size_t l = 0; l < m.images.size(); l++) {
+                std::printf("%s[", l ? "," : "");
+                for (int i = 0; i < m.images[l].width * m.images[l].height; i++) {
+                    if (i) std::printf(",");
+                    jd(m.images[l](i));
+                }
+                std::printf("]");
+            }
+        }
+        std::printf("]}");
+    }
+    std::printf("],\"checker\":[");
+    for (int q = 0; q < 64; q++) {
+        Vector3 c0{U(), U(), U()}, c1{U(), U(), U()};
+        double us = 0.5 + 4 * U(), vs = 0.5 + 4 * U(), uo = 2 * U() - 1, vo = 2 * U() - 1;
+        Vector2 uv{4 * U() - 2, 4 * U() - 2};
+        CheckerboardTexture<Spectrum> t{c0, c1, us, vs, uo, vo};
+        Vector3 out = eval(Texture<Spectrum>{t}, uv, Real(0), pool);
+        CheckerboardTexture<Real> t1{avg(c0), avg(c1), us, vs, uo, vo};
+        Real out1 = eval(Texture<Real>{t1}, uv, Real(0), pool);
+        std::printf("%s{\"color0\":", q ? "," : "");
+        jv(c0); std::printf(",\"color1\":"); jv(c1);
+        std::printf(",\"scale\":["); jd(us); std::printf(","); jd(vs);
+        std::printf("],\"offset\":["); jd(uo); std::printf(","); jd(vo);
+        std::printf("],\"uv\":["); jd(uv.x); std::printf(","); jd(uv.y);
+        std::printf("],\"out\":"); jv(out); std::printf(",\"out1\":"); jd(out1); std::printf("}");
+    }
+    std::printf("]}\n");
+}
+
 int main(int argc, char **argv) {
     std::string mode = argc > 1 ? argv[1] : "";
     if (mode == "pcg") kat_pcg();
@@ -292,8 +385,9 @@ int main(int argc, char **argv) {
     else if (mode == "camera") kat_camera();
     else if (mode == "materials") kat_materials();
     else if (mode == "phase") kat_phase();
+    else if (mode == "texture") kat_texture();
     else {
-        std::fprintf(stderr, "usage: ref_kat pcg|table|camera|materials|phase\n");
+        std::fprintf(stderr, "usage: ref_kat pcg|table|camera|materials|phase|texture\n");
         return 2;
     }
     return 0;

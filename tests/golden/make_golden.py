@@ -21,7 +21,7 @@ KAT = os.path.join(ROOT, "oracle", "_ref", "ref_kat")
 
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
-    for mode in ("pcg", "table", "camera", "materials", "phase"):
+    for mode in ("pcg", "table", "camera", "materials", "phase", "texture"):
         out = subprocess.run([KAT, mode], check=True, capture_output=True, text=True).stdout
         data = json.loads(out)
         path = os.path.join(HERE, f"ref_{mode}.json")

@@ -30,12 +30,14 @@ def test_library_exports_every_symbol():
     assert lib.ljx_abi_version() == 1
 
 
-STRUCTS = ["ljx_shape_desc", "ljx_material_desc", "ljx_light_desc", "ljx_medium_desc", "ljx_camera_desc",
+STRUCTS = ["ljx_shape_desc", "ljx_material_desc", "ljx_light_desc", "ljx_medium_desc", "ljx_volume_desc",
+           "ljx_texture_desc", "ljx_camera_desc",
            "ljx_render_options", "ljx_scene_desc", "ljx_ray", "ljx_hit", "ljx_bsdf_query", "ljx_bsdf_result",
            "ljx_render_stats", "ljx_exec_options", "ljx_guide_config", "ljx_guide_options", "ljx_batch_job",
            "ljx_batch_stats"]
 MIRROR = {"ljx_shape_desc": capi.ShapeDesc, "ljx_material_desc": capi.MaterialDesc, "ljx_light_desc": capi.LightDesc,
-          "ljx_medium_desc": capi.MediumDesc, "ljx_camera_desc": capi.CameraDesc,
+          "ljx_medium_desc": capi.MediumDesc, "ljx_volume_desc": capi.VolumeDesc,
+          "ljx_texture_desc": capi.TextureDesc, "ljx_camera_desc": capi.CameraDesc,
           "ljx_render_options": capi.RenderOptions, "ljx_scene_desc": capi.SceneDescC, "ljx_ray": capi.Ray,
           "ljx_hit": capi.Hit, "ljx_bsdf_query": capi.BSDFQuery, "ljx_bsdf_result": capi.BSDFResult,
           "ljx_render_stats": capi.RenderStats, "ljx_exec_options": capi.ExecOptions,
