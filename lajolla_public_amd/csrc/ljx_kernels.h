@@ -348,24 +348,25 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
         double lw = pcg_double(st, inc), sw = pcg_double(st, inc);
         int lid = sample_cdf(S.light_cdf, S.num_lights, lw);
         const LightRec &l = S.lights[lid];
-        LightPoint lp = sample_light_point(S, l, v.pos, lu0, lu1, sw);
-        D3 dir_light;
-        double G = 0, tfar;
+        D3 dir_light, L;
+        double G = 0, tfar, lpdf;
         if (l.type == 0) {
+            LightPoint lp = sample_shape(S, S.shapes[l.shape_id], v.pos, lu0, lu1, sw);
             dir_light = normalize(lp.pos - v.pos);
             double dl = sqrt(dist2(lp.pos, v.pos));
             tfar = (1 - S.eps) * dl;
             G = fmax(-dot(dir_light, lp.n), 0.0) / dist2(lp.pos, v.pos);
+            lpdf = pdf_shape(S.shapes[l.shape_id], lp, v.pos);
+            L = dot(lp.n, -dir_light) > 0 ? l.intensity : D3{0, 0, 0};
         } else {
-            dir_light = -lp.n;
+            env_nee(l, lu0, lu1, dir_light, lpdf, L);
             tfar = __builtin_huge_val();
             G = 1;
         }
-        double p1 = S.light_pmf[lid] * pdf_light_point(S, l, lp, v.pos);
+        double p1 = S.light_pmf[lid] * lpdf;
         D3 dir_view = -dir;
         if (G > 0 && p1 > 0) {
             D3 f = bsdf_eval<MT>(mat, dir_view, dir_light, v.g);
-            D3 L = light_emission(S, l, -dir_light, lp);
             D3 C1 = G * f * L;
             double pb = bsdf_pdf<MT>(mat, dir_view, dir_light, v.g);
             if (guided)
@@ -430,7 +431,7 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
 
 // MT = -1: the unsorted bucket, with the material type dispatched per path.
 template <int MT, bool SPH, bool GUIDE>
-__global__ __launch_bounds__(256) void k_shade(DeviceScene S, PathBufs P, int cur) {
+__device__ __forceinline__ void shade_queue(const DeviceScene &S, const PathBufs &P, int cur) {
     constexpr int B = MT >= 0 ? MT : kBucketAny;
     int n = P.ctr[C_BUCKET + B];
     const uint32_t *q = P.bucket + size_t(B) * P.capacity;
@@ -457,6 +458,21 @@ __global__ __launch_bounds__(256) void k_shade(DeviceScene S, PathBufs P, int cu
     }
 }
 
+// The shading kernel of one material type (or, MT = -1, of the runtime-dispatch
+// bucket).  Occupancy of these FP64-heavy kernels is set by their register
+// count: the 2-wave variant caps it at 256 VGPRs for the material types whose
+// code fits there with little or no spilling (Lambertian, RoughPlastic,
+// DisneyDiffuse, DisneyMetal, DisneySheen); the others run one wave per SIMD.
+template <int MT, bool SPH, bool GUIDE>
+__global__ __launch_bounds__(256) void k_shade(DeviceScene S, PathBufs P, int cur) {
+    shade_queue<MT, SPH, GUIDE>(S, P, cur);
+}
+template <int MT, bool SPH, bool GUIDE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_shade_w2(DeviceScene S, PathBufs P,
+                                                                                         int cur) {
+    shade_queue<MT, SPH, GUIDE>(S, P, cur);
+}
+
 // Escaped paths: environment emission with MIS (path_tracing.h:17-28, 284-302).
 __global__ __launch_bounds__(256) void k_miss(DeviceScene S, PathBufs P) {
     int n = P.ctr[C_BUCKET + kBucketMiss];
@@ -472,7 +488,12 @@ __global__ __launch_bounds__(256) void k_miss(DeviceScene S, PathBufs P) {
         D3 tp{tp4.x, tp4.y, tp4.z};
         float4 r4 = P.rad[pid];
         D3 rad{r4.x, r4.y, r4.z};
-        D3 L = light_emission(S, env, -dir, LightPoint{{0, 0, 0}, {0, 0, 0}});
+        // the direction's lat-long coordinates serve the radiance and the pdf
+        // (src/lights/envmap.inl:22-72)
+        const D3 ld = xform3(env.to_local, dir);
+        double eu, ev;
+        env_uv(ld, eu, ev);
+        D3 L = env_texel_bilinear(env, eu, ev) * env.scale;
         D3 rad_in = rad;
         int b = P.depth[pid];
         if (b == 0) {
@@ -482,8 +503,9 @@ __global__ __launch_bounds__(256) void k_miss(DeviceScene S, PathBufs P) {
             D3 f{pv.x, pv.y, pv.z};
             double p2 = pv.w;  // G = 1
             D3 C2 = f * L;
-            LightPoint lp{{0, 0, 0}, -dir};
-            double p1 = S.light_pmf[S.envmap_light_id] * pdf_light_point(S, env, lp, D3{0, 0, 0});
+            const double se = sqrt(clampd(1 - ld.y * ld.y, 0, 1));
+            const double epdf = se > 0 ? pdf_env_uv(env, eu, ev) / (2 * kPi * kPi * se) : 0.0;
+            double p1 = S.light_pmf[S.envmap_light_id] * epdf;
             double w2 = (p2 * p2) / (p1 * p1 + p2 * p2);
             C2 = C2 / p2;
             rad = rad + tp * C2 * w2;

@@ -886,14 +886,14 @@ template <bool SPH, bool GUIDE>
 void shade_all_t(Scene &sc, PathBufs &P, int cur, int grid, int block, hipStream_t s) {
     const DeviceScene &S = sc.ds;
     unsigned m = sc.mat_types_present;
-    if (m & (1u << 0)) launch(k_shade<0, SPH, GUIDE>, grid, block, s, S, P, cur);
-    if (m & (1u << 1)) launch(k_shade<1, SPH, GUIDE>, grid, block, s, S, P, cur);
+    if (m & (1u << 0)) launch(k_shade_w2<0, SPH, GUIDE>, grid, block, s, S, P, cur);
+    if (m & (1u << 1)) launch(k_shade_w2<1, SPH, GUIDE>, grid, block, s, S, P, cur);
     if (m & (1u << 2)) launch(k_shade<2, SPH, GUIDE>, grid, block, s, S, P, cur);
-    if (m & (1u << 3)) launch(k_shade<3, SPH, GUIDE>, grid, block, s, S, P, cur);
-    if (m & (1u << 4)) launch(k_shade<4, SPH, GUIDE>, grid, block, s, S, P, cur);
+    if (m & (1u << 3)) launch(k_shade_w2<3, SPH, GUIDE>, grid, block, s, S, P, cur);
+    if (m & (1u << 4)) launch(k_shade_w2<4, SPH, GUIDE>, grid, block, s, S, P, cur);
     if (m & (1u << 5)) launch(k_shade<5, SPH, GUIDE>, grid, block, s, S, P, cur);
     if (m & (1u << 6)) launch(k_shade<6, SPH, GUIDE>, grid, block, s, S, P, cur);
-    if (m & (1u << 7)) launch(k_shade<7, SPH, GUIDE>, grid, block, s, S, P, cur);
+    if (m & (1u << 7)) launch(k_shade_w2<7, SPH, GUIDE>, grid, block, s, S, P, cur);
     if (m & (1u << 8)) launch(k_shade<8, SPH, GUIDE>, grid, block, s, S, P, cur);
     // unsorted bounces and textured materials go to the runtime-dispatch kernel
     if (P.sort_threshold > 0 || sc.has_textured) launch(k_shade<-1, SPH, GUIDE>, grid, block, s, S, P, cur);

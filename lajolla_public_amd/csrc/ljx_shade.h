@@ -305,6 +305,36 @@ __device__ __forceinline__ double pdf_light_point(const DeviceScene &S, const Li
     if (l.type == 1) return pdf_env(l, -lp.n);
     return pdf_shape(S.shapes[l.shape_id], lp, ref);
 }
+// Level-0 bilinear radiance of the envmap at lat-long (u, v) (src/mipmap.h:50-72; its
+// texels are the envmap's FP32 values).
+__device__ __forceinline__ D3 env_texel_bilinear(const LightRec &l, double u, double v) {
+    const int W = l.env_w, H = l.env_h;
+    u = u * W - 0.5;
+    v = v * H - 0.5;
+    const int ufi = imod(int(u), W), vfi = imod(int(v), H);
+    const int uci = imod(ufi + 1, W), vci = imod(vfi + 1, H);
+    const double uo = u - ufi, vo = v - vfi;
+    auto tx = [&](int x, int y) {
+        const float *p = l.env_rgb + 3 * (size_t(y) * W + x);
+        return D3{double(p[0]), double(p[1]), double(p[2])};
+    };
+    return tx(ufi, vfi) * (1 - uo) * (1 - vo) + tx(ufi, vci) * (1 - uo) * vo + tx(uci, vfi) * uo * (1 - vo) +
+           tx(uci, vci) * uo * vo;
+}
+// Next-event estimation towards the envmap (src/lights/envmap.inl:7-72): the
+// TableDist2D sample's lat-long coordinates give the direction, its pdf and
+// the radiance without mapping the direction back to (u, v).
+__device__ __forceinline__ void env_nee(const LightRec &l, double u0, double u1, D3 &dir, double &pdf, D3 &L) {
+    double eu, ev;
+    sample_env_uv(l, u0, u1, eu, ev);
+    double sa, ca, se, ce;
+    sincos(eu * (2 * kPi), &sa, &ca);
+    sincos(ev * kPi, &se, &ce);
+    dir = xform3(l.to_world, D3{sa * se, ce, -ca * se});
+    pdf = se > 0 ? pdf_env_uv(l, eu, ev) / (2 * kPi * kPi * se) : 0.0;
+    L = env_texel_bilinear(l, eu, ev) * l.scale;
+}
+
 // emission (src/lights/diffuse_area_light.inl:15-20; src/lights/envmap.inl:44-72).
 // The envmap's footprint, min(|du/dw|, dv/dw), is never positive (dv/dw < 0),
 // so the reference reads the image bilinearly on level 0; only along the exact
@@ -312,23 +342,10 @@ __device__ __forceinline__ double pdf_light_point(const DeviceScene &S, const Li
 __device__ __forceinline__ D3 light_emission(const DeviceScene &S, const LightRec &l, D3 view_dir,
                                             const LightPoint &lp) {
     if (l.type == 1) {
-        // mip_lookup on level 0, whose texels are the envmap's FP32 values (src/mipmap.h:50-72)
         D3 w = xform3(l.to_local, -view_dir);
         double u, v;
         env_uv(w, u, v);
-        const int W = l.env_w, H = l.env_h;
-        u = u * W - 0.5;
-        v = v * H - 0.5;
-        const int ufi = imod(int(u), W), vfi = imod(int(v), H);
-        const int uci = imod(ufi + 1, W), vci = imod(vfi + 1, H);
-        const double uo = u - ufi, vo = v - vfi;
-        auto tx = [&](int x, int y) {
-            const float *p = l.env_rgb + 3 * (size_t(y) * W + x);
-            return D3{double(p[0]), double(p[1]), double(p[2])};
-        };
-        D3 c = tx(ufi, vfi) * (1 - uo) * (1 - vo) + tx(ufi, vci) * (1 - uo) * vo + tx(uci, vfi) * uo * (1 - vo) +
-               tx(uci, vci) * uo * vo;
-        return c * l.scale;
+        return env_texel_bilinear(l, u, v) * l.scale;
     }
     if (dot(lp.n, view_dir) <= 0) return {0, 0, 0};
     return l.intensity;
