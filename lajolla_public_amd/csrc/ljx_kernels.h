@@ -33,6 +33,7 @@ constexpr int kNumBuckets = 12;
 constexpr int C_EXT = 0;      // C_EXT + (0|1): extension queues
 constexpr int C_SH = 2;       // shadow queue
 constexpr int C_BUCKET = 4;   // C_BUCKET + bucket
+constexpr int C_XFETCH = 16;  // C_XFETCH + (0|1): rays taken from extension queue 0|1 by the traversal
 constexpr int C_TOTAL = 16 + 4;
 
 // One job of a batched render: its camera, image and place in the batch's
@@ -63,6 +64,7 @@ struct PathBufs {
     uint32_t *bucket; // kNumBuckets * capacity
     uint32_t *shq;
     float4 *sh_o, *sh_d, *sh_c;
+    uint8_t *sh_occ;  // per shadow ray: 1 when occluded
     // volumetric path state
     int *medium;      // current medium id
     float4 *nee_c;    // nee_p_cache xyz, dir_pdf
@@ -199,39 +201,53 @@ __global__ void k_raygen(DeviceScene S, PathBufs P, int compact) {
 }
 
 // Closest-hit traversal over the extension queue, then material bucketing.
+// Closest-hit traversal of the extension queue.  A wave takes 64 rays at a
+// time with one atomic, so waves that drew short rays take more (a launch
+// has only a few rays per lane); each hit's distance is refined while its
+// triangle is in cache, and the wave appends its paths to the buckets of
+// the materials they hit (the material sort) with one atomic per bucket.
+// The fetch counter of a launch starts at zero: k_reset_counters clears
+// every counter at a pass's start and each launch clears the one the next
+// launch uses.
 template <bool STATS, bool SPH>
 __global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int cur) {
     __shared__ int stack[kStackLds * 256];
     const uint32_t *q = cur ? P.extq1 : P.extq0;
-    int n = P.ctr[C_EXT + cur];
+    const int n = P.ctr[C_EXT + cur];
     const bool sorted = n >= P.sort_threshold;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         P.ctr[C_EXT + (cur ^ 1)] = 0;
         P.ctr[C_SH] = 0;
+        P.ctr[C_XFETCH + (cur ^ 1)] = 0;
         atomicAdd(&P.stat[S_EXT_RAYS], (unsigned long long)n);
     }
-    int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    int stride = gridDim.x * blockDim.x;
+    int *fetch = &P.ctr[C_XFETCH + cur];
+    const TraceScene ts{S.nodes, S.prims, S.spheres};
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int sstride = gridDim.x * blockDim.x;
     int *spill = P.spill + gtid;
+    const int lane = __lane_id();
     uint32_t nn = 0, np = 0;
-    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
-        int i = base + threadIdx.x;
+    while (true) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(fetch, 64);
+        base = __shfl(base, 0);
+        if (base >= n) break;
+        const int i = base + lane;
         int bucket = -1;
         uint32_t pid = 0;
         if (i < n) {
             pid = q[i];
-            float4 o = P.ray_o[pid], d = P.ray_d[pid];
+            const float4 o = P.ray_o[pid], d = P.ray_d[pid];
             TraceHit h;
-            bool hit = S.num_nodes > 0 &&
-                       trace_ray<false, STATS, SPH>(TraceScene{S.nodes, S.prims, S.spheres}, o.x, o.y, o.z, d.x, d.y, d.z, o.w, d.w,
-                                               stack + threadIdx.x, blockDim.x, spill, stride, h, nn, np);
+            const bool hit = S.num_nodes > 0 && trace_ray<false, STATS, SPH>(ts, o.x, o.y, o.z, d.x, d.y, d.z, o.w, d.w,
+                                                                             stack + threadIdx.x, blockDim.x, spill,
+                                                                             sstride, h, nn, np);
             if (hit) {
                 P.hit[pid] = make_float4(h.t, h.u, h.v, __int_as_float(h.slot));
-                int gid = __float_as_int(S.prims[3 * h.slot].w);
-                int mid = S.shapes[S.prim_shape[gid]].material_id;
-                // textured materials are shaded by the runtime-dispatch kernel, which evaluates them
-                bucket = mid < 0 ? kBucketNoMat
-                                 : (sorted && !S.materials[mid].textured ? int(S.materials[mid].type) : kBucketAny);
+                const int sb = S.slot_bucket[h.slot];
+                // textured materials are shaded by the runtime-dispatch kernel
+                bucket = (sorted || sb >= kNumMatTypes) ? sb : kBucketAny;
             } else {
                 P.hit[pid] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
                 bucket = kBucketMiss;
@@ -240,16 +256,8 @@ __global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int c
             // free-flight sampling: one queue
             if (P.volpath) bucket = 0;
         }
-        // one append per distinct bucket present in the wave
-        uint64_t pending = __ballot(bucket >= 0);
-        while (pending) {
-            int leader = __ffsll((unsigned long long)pending) - 1;
-            int b = __shfl(bucket, leader);
-            bool mine = bucket == b;
-            int k = wave_append(mine, &P.ctr[C_BUCKET + b]);
-            if (mine) P.bucket[size_t(b) * P.capacity + k] = pid;
-            pending &= ~__ballot(mine);
-        }
+        const int k = wave_append_keyed(bucket, &P.ctr[C_BUCKET]);
+        if (bucket >= 0) P.bucket[size_t(bucket) * P.capacity + k] = pid;
     }
     if (STATS) {
         atomicAdd(&P.stat[S_NODES], (unsigned long long)nn);
@@ -518,41 +526,54 @@ __global__ __launch_bounds__(256) void k_miss(DeviceScene S, PathBufs P) {
     }
 }
 
+// Any-hit traversal of the shadow queue: each lane runs its own rays one
+// after another (no wave-level step between them) and writes whether each is
+// occluded; k_shadow_resolve adds the contributions.
 template <bool STATS, bool SPH>
 __global__ __launch_bounds__(256) void k_shadow(DeviceScene S, PathBufs P) {
     __shared__ int stack[kStackLds * 256];
-    int n = P.ctr[C_SH];
+    const int n = P.ctr[C_SH];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(&P.stat[S_SH_RAYS], (unsigned long long)n);
         for (int b = 0; b < kNumBuckets; b++) P.ctr[C_BUCKET + b] = 0;
     }
-    int gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    int stride = gridDim.x * blockDim.x;
+    const TraceScene ts{S.nodes, S.prims, S.spheres};
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
     int *spill = P.spill + gtid;
     uint32_t nn = 0, np = 0;
     for (int i = gtid; i < n; i += stride) {
-        float4 o = P.sh_o[i], d = P.sh_d[i];
+        const float4 o = P.sh_o[i], d = P.sh_d[i];
         TraceHit h;
-        bool occ = S.num_nodes > 0 &&
-                   trace_ray<true, STATS, SPH>(TraceScene{S.nodes, S.prims, S.spheres}, o.x, o.y, o.z, d.x, d.y, d.z, o.w, d.w,
-                                          stack + threadIdx.x, blockDim.x, spill, stride, h, nn, np);
-        if (!occ) {
-            float4 c = P.sh_c[i];
-            uint32_t bits = uint32_t(__float_as_int(c.w));
-            uint32_t pid = bits & kSlotMask;
-            float4 r = P.rad[pid];
-            P.rad[pid] = make_float4(r.x + c.x, r.y + c.y, r.z + c.z, 0.f);
-            int b = int(bits >> kSlotBits);
-            if (b < P.rec_depth) {
-                float4 *rc = &P.rec_c[size_t(b) * P.rec_stride + pid];
-                float4 o = *rc;
-                *rc = make_float4(o.x + c.x, o.y + c.y, o.z + c.z, 0.f);
-            }
-        }
+        const bool occ = S.num_nodes > 0 && trace_ray<true, STATS, SPH>(ts, o.x, o.y, o.z, d.x, d.y, d.z, o.w, d.w,
+                                                                         stack + threadIdx.x, blockDim.x, spill, stride,
+                                                                         h, nn, np);
+        P.sh_occ[i] = occ ? 1 : 0;
     }
     if (STATS) {
         atomicAdd(&P.stat[S_SH_NODES], (unsigned long long)nn);
         atomicAdd(&P.stat[S_SH_PRIMS], (unsigned long long)np);
+    }
+}
+
+// After k_shadow: an unoccluded shadow ray adds its pending NEE contribution
+// to its path (and to the recorded radiance of its vertex when guiding
+// records the pass).
+__global__ __launch_bounds__(256) void k_shadow_resolve(PathBufs P) {
+    const int n = P.ctr[C_SH];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (P.sh_occ[i]) continue;
+        const float4 c = P.sh_c[i];
+        const uint32_t bits = uint32_t(__float_as_int(c.w));
+        const uint32_t pid = bits & kSlotMask;
+        const float4 r = P.rad[pid];
+        P.rad[pid] = make_float4(r.x + c.x, r.y + c.y, r.z + c.z, 0.f);
+        const int b = int(bits >> kSlotBits);
+        if (b < P.rec_depth) {
+            float4 *rc = &P.rec_c[size_t(b) * P.rec_stride + pid];
+            const float4 o = *rc;
+            *rc = make_float4(o.x + c.x, o.y + c.y, o.z + c.z, 0.f);
+        }
     }
 }
 

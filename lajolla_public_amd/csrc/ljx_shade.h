@@ -23,6 +23,28 @@ __device__ __forceinline__ int wave_append(bool pred, int *counter) {
     return pred ? base + below : -1;
 }
 
+// Lanes with key >= 0 append to the queue counters[key]: one atomic per
+// distinct key, all issued by one instruction (the lowest lane of each key
+// adds its key's count).  Returns the lane's index in its queue, or -1.
+__device__ __forceinline__ int wave_append_keyed(int key, int *counters) {
+    uint64_t pending = __ballot(key >= 0);
+    uint64_t mine = 0;
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int k = __builtin_amdgcn_readlane(key, leader);
+        const uint64_t m = __ballot(key == k);
+        if (key == k) mine = m;
+        pending &= ~m;
+    }
+    const int lane = __lane_id();
+    const int first = mine ? __ffsll((unsigned long long)mine) - 1 : lane;
+    int base = 0;
+    if (key >= 0 && lane == first) base = atomicAdd(&counters[key], __popcll(mine));
+    base = __shfl(base, first);
+    const int below = __builtin_amdgcn_mbcnt_hi(uint32_t(mine >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mine), 0));
+    return key >= 0 ? base + below : -1;
+}
+
 __device__ __forceinline__ D3 f3d(float4 v) { return D3{double(v.x), double(v.y), double(v.z)}; }
 
 // ------------------------------------------------------------- path vertex

@@ -175,39 +175,49 @@ struct TraceScene {
     const double4 *__restrict__ spheres; // centre xyz, radius
 };
 
-// Trace one ray.  stack: this lane's LDS column base (entries at stack[k*stride]);
-// spill: this lane's global spill column.  Returns true on a hit.  SPH: the
-// scene has sphere primitives (otherwise every leaf slot is a triangle).
+// Traversal state of one ray, advanced one inner node or one leaf per step.
+// stack: this lane's LDS column base (entries at stack[k*stride]); spill:
+// this lane's global spill column.  SPH: the scene has sphere primitives
+// (otherwise every leaf slot is a triangle).
 template <bool ANY, bool STATS, bool SPH>
-__device__ __forceinline__ bool trace_ray(const TraceScene &ts, float ox, float oy, float oz, float dx, float dy,
-                                          float dz, float tmin, float tmax, int *stack, int stride, int *spill,
-                                          int spill_stride, TraceHit &hit, uint32_t &n_nodes, uint32_t &n_prims) {
-    const Bvh4Node *__restrict__ nodes = ts.nodes;
-    const float4 *__restrict__ prims = ts.prims;
-    const float kMiss = __builtin_huge_valf();
-    RayPre r = ray_setup(ox, oy, oz, dx, dy, dz);
-    int sp = 0;
-    int node = 0;  // the root is an inner node
-    bool found = false;
-    hit.slot = -1;
-    auto push = [&](int v) {
-        if (sp < kStackLds) stack[sp * stride] = v;
-        else spill[(sp - kStackLds) * spill_stride] = v;
-        sp++;
-    };
-    auto cswap = [](float &ta, int &ca, float &tb, int &cb) {
-        if (tb < ta) {
-            float t = ta;
-            ta = tb;
-            tb = t;
-            int c = ca;
-            ca = cb;
-            cb = c;
-        }
-    };
-    while (true) {
+struct Tracer {
+    RayPre r;
+    float dx, dy, dz, tmin, tmax;
+    int node, sp;
+    bool found;
+    TraceHit hit;
+
+    __device__ __forceinline__ void start(float ox, float oy, float oz, float dx_, float dy_, float dz_, float tmin_,
+                                          float tmax_) {
+        r = ray_setup(ox, oy, oz, dx_, dy_, dz_);
+        dx = dx_;
+        dy = dy_;
+        dz = dz_;
+        tmin = tmin_;
+        tmax = tmax_;
+        node = 0;  // the root is an inner node
+        sp = 0;
+        found = false;
+        hit.slot = -1;
+    }
+
+    // One inner node or one leaf.  Returns true when the ray is finished: the
+    // stack ran empty, or (any hit) an occluder was found.
+    __device__ __forceinline__ bool step(const TraceScene &ts, int *stack, int stride, int *spill, int spill_stride,
+                                         uint32_t &n_nodes, uint32_t &n_prims) {
+        const float kMiss = __builtin_huge_valf();
+        auto cswap = [](float &ta, int &ca, float &tb, int &cb) {
+            if (tb < ta) {
+                float t = ta;
+                ta = tb;
+                tb = t;
+                int c = ca;
+                ca = cb;
+                cb = c;
+            }
+        };
         if (node >= 0) {
-            const Bvh4Node *nd = nodes + node;
+            const Bvh4Node *nd = ts.nodes + node;
             float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
             int4 ch = nd->ch;
             if (STATS) n_nodes++;
@@ -218,41 +228,43 @@ __device__ __forceinline__ bool trace_ray(const TraceScene &ts, float ox, float 
             int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
             if (c2 == kNoChild) t2 = kMiss;  // empty slots carry inverted boxes, which
             if (c3 == kNoChild) t3 = kMiss;  // a slab test alone does not reject
-            if (ANY) {
-                // any hit: only the first child to visit matters (the nearest);
-                // the others are pushed in whatever order, each if it was hit
-                cswap(t0, c0, t1, c1);
-                cswap(t2, c2, t3, c3);
-                cswap(t0, c0, t2, c2);
-            } else {
-                cswap(t0, c0, t1, c1);
-                cswap(t2, c2, t3, c3);
-                cswap(t0, c0, t2, c2);
+            // three comparators put the nearest child first; any hit needs no
+            // more (the others are pushed in whatever order, each if it was
+            // hit), closest hit orders all four with two more
+            cswap(t0, c0, t1, c1);
+            cswap(t2, c2, t3, c3);
+            cswap(t0, c0, t2, c2);
+            if (!ANY) {
                 cswap(t1, c1, t3, c3);
                 cswap(t1, c1, t2, c2);
             }
             if (t0 != kMiss) {
+                auto push = [&](int v) {
+                    if (sp < kStackLds) stack[sp * stride] = v;
+                    else spill[(sp - kStackLds) * spill_stride] = v;
+                    sp++;
+                };
                 if (t3 != kMiss) push(c3);
                 if (t2 != kMiss) push(c2);
                 if (t1 != kMiss) push(c1);
                 node = c0;
-                continue;
+                return false;
             }
         } else {
-            int code = ~node;
-            int first = code >> 3, cnt = (code & 7) + 1;
+            const int code = ~node;
+            const int first = code >> 3, cnt = (code & 7) + 1;
             for (int k = 0; k < cnt; k++) {
-                int slot = first + k;
-                float4 p0 = prims[3 * slot], p1 = prims[3 * slot + 1];
+                const int slot = first + k;
+                float4 p0 = ts.prims[3 * slot], p1 = ts.prims[3 * slot + 1];
                 float t, u = 0, v = 0;
                 bool ok;
                 if (STATS) n_prims++;
                 if (!SPH || __float_as_int(p1.w) == kPrimTri) {
-                    float4 p2 = prims[3 * slot + 2];
+                    float4 p2 = ts.prims[3 * slot + 2];
                     ok = tri_test(r, p0, p1, p2, tmin, tmax, t, u, v);
                 } else {
                     double td;
-                    ok = sphere_roots(ox, oy, oz, dx, dy, dz, ts.spheres[__float_as_int(p1.x)], tmin, tmax, td);
+                    ok = sphere_roots(r.ox, r.oy, r.oz, dx, dy, dz, ts.spheres[__float_as_int(p1.x)], tmin, tmax, td);
                     t = float(td);
                 }
                 if (ok) {
@@ -266,29 +278,48 @@ __device__ __forceinline__ bool trace_ray(const TraceScene &ts, float ox, float 
                 }
             }
         }
-        if (sp == 0) break;
+        if (sp == 0) return true;
         sp--;
         node = sp < kStackLds ? stack[sp * stride] : spill[(sp - kStackLds) * spill_stride];
+        return false;
     }
-    if (!ANY && found) {
-        float4 p0 = prims[3 * hit.slot], p1 = prims[3 * hit.slot + 1];
-        if (!SPH || __float_as_int(p1.w) == kPrimTri) {
-            // The FP32 distance loses relative accuracy when the hit is close
-            // to the origin next to a large triangle (T cancels); recompute
-            // it once in FP64 from the triangle's plane.
-            float4 p2 = prims[3 * hit.slot + 2];
-            double e1x = double(p1.x) - p0.x, e1y = double(p1.y) - p0.y, e1z = double(p1.z) - p0.z;
-            double e2x = double(p2.x) - p0.x, e2y = double(p2.y) - p0.y, e2z = double(p2.z) - p0.z;
-            double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
-            double den = nx * dx + ny * dy + nz * dz;
-            if (den != 0.0) {
-                double num = nx * (double(p0.x) - ox) + ny * (double(p0.y) - oy) + nz * (double(p0.z) - oz);
-                hit.t = float(num / den);
-            }
+
+};
+
+// Closest hit on a triangle: the FP32 distance loses relative accuracy when
+// the hit is close to the origin next to a large triangle (T cancels);
+// recompute it once in FP64 from the triangle's plane.  Sphere hits keep
+// their distance (sphere_hit_uv() re-solves them).
+template <bool SPH>
+__device__ __forceinline__ float refine_t(const float4 *__restrict__ prims, int slot, float t, float ox, float oy,
+                                          float oz, float dx, float dy, float dz) {
+    float4 p0 = prims[3 * slot], p1 = prims[3 * slot + 1];
+    if (!SPH || __float_as_int(p1.w) == kPrimTri) {
+        float4 p2 = prims[3 * slot + 2];
+        double e1x = double(p1.x) - p0.x, e1y = double(p1.y) - p0.y, e1z = double(p1.z) - p0.z;
+        double e2x = double(p2.x) - p0.x, e2y = double(p2.y) - p0.y, e2z = double(p2.z) - p0.z;
+        double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+        double den = nx * dx + ny * dy + nz * dz;
+        if (den != 0.0) {
+            double num = nx * (double(p0.x) - ox) + ny * (double(p0.y) - oy) + nz * (double(p0.z) - oz);
+            t = float(num / den);
         }
-        // sphere hits leave u, v to sphere_hit_uv()
     }
-    return found;
+    return t;
+}
+
+// Trace one ray to the end.  Returns true on a hit.
+template <bool ANY, bool STATS, bool SPH>
+__device__ __forceinline__ bool trace_ray(const TraceScene &ts, float ox, float oy, float oz, float dx, float dy,
+                                          float dz, float tmin, float tmax, int *stack, int stride, int *spill,
+                                          int spill_stride, TraceHit &hit, uint32_t &n_nodes, uint32_t &n_prims) {
+    Tracer<ANY, STATS, SPH> tr;
+    tr.start(ox, oy, oz, dx, dy, dz, tmin, tmax);
+    while (!tr.step(ts, stack, stride, spill, spill_stride, n_nodes, n_prims)) {
+    }
+    hit = tr.hit;
+    if (!ANY && tr.found) hit.t = refine_t<SPH>(ts.prims, hit.slot, hit.t, ox, oy, oz, dx, dy, dz);
+    return tr.found;
 }
 
 // For a sphere hit: the FP64 distance and the spherical coordinates used as
