@@ -70,6 +70,8 @@ struct PathBufs {
     float4 *nee_c;    // nee_p_cache xyz, dir_pdf
     float4 *mtp;      // multi_trans_pdf rgb, never_scatter (1/0)
     float4 *sh_x;     // shadow walk: pdf_nee, pdf_dir, NEE stream state (lo, hi bits)
+    double4 *vf_t;    // free flight: transmittance rgb, distance travelled
+    double4 *vf_p;    // free flight: transmittance pdf rgb, 1 if it ended in a real collision
     int *ctr;
     unsigned long long *stat;
     int *spill;
@@ -80,6 +82,7 @@ struct PathBufs {
     int pass_s0, pass_ns, pass_q0, pass_nq;
     int spp, max_depth, rr_depth;
     int volpath;
+    int vol_flight;  // free flight runs in k_vol_flight (scenes with grid media), else inside k_vol_shade
     int max_null;
     uint64_t seed;
     // learned guiding (path integrator)

@@ -202,6 +202,7 @@ struct Scene {
     unsigned mat_types_present = 0;
     bool has_textured = false;  // some material reads a texture: shaded by the runtime-dispatch kernel
     bool has_media = false;
+    bool has_grid_media = false;  // a heterogeneous medium: long delta-tracking loops
     int num_media = 0;
     double bound_center[3] = {0, 0, 0};
     double bound_radius = 0;
@@ -366,6 +367,8 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
     sc.height = d->camera.height;
     sc.has_media = d->num_media > 0;
     sc.num_media = d->num_media;
+    sc.has_grid_media = false;
+    for (int i = 0; i < d->num_media; i++) sc.has_grid_media = sc.has_grid_media || d->media[i].type == LJX_MEDIUM_HETEROGENEOUS;
 
     // ---- flatten geometry
     std::vector<float> vpos, vnrm, vuv;
@@ -1069,6 +1072,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
     P.rr_depth = opt.rr_depth;
     P.seed = opt.seed;
     P.volpath = vol ? 1 : 0;
+    P.vol_flight = vol && sc.has_grid_media ? 1 : 0;
     P.max_null = opt.max_null_collisions;
     P.rec_depth = 0;
     P.jobs = nullptr;
@@ -1422,6 +1426,7 @@ void batch_impl(const ljx_batch_job *jobs, size_t njobs, const ljx_exec_options 
             P.rr_depth = sc.options.rr_depth;
             P.seed = 0;
             P.volpath = vol ? 1 : 0;
+            P.vol_flight = vol && sc.has_grid_media ? 1 : 0;
             P.max_null = sc.options.max_null_collisions;
             P.rec_depth = 0;
             P.jobs = gp.dtable;

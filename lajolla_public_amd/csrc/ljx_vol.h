@@ -53,6 +53,87 @@ __device__ __forceinline__ int update_medium(D3 dir, const Vertex &v, int medium
     return medium;
 }
 
+// Free-flight sampling of one path in a medium (homework2.tex:745-769, delta
+// tracking; its first draw picks the channel).  It is the first use of the
+// path's stream in a bounce, so in scenes with grid media (long loops of
+// null collisions) k_vol_flight runs it for every path of the queue before
+// k_vol_shade, which takes over the stream state; otherwise k_vol_shade runs
+// it in place.
+struct Flight {
+    D3 T, trans_dir_pdf;
+    double accum_t;
+    bool scatter;
+};
+__device__ __forceinline__ Flight free_flight(const DeviceScene &S, const PathBufs &P, int medium, D3 org, D3 dir,
+                                              double t_hit, uint64_t &st, uint64_t inc) {
+    Flight fl{{1, 1, 1}, {1, 1, 1}, 0.0, false};
+    const MediumRec &m = S.media[medium];
+    D3 maj = medium_majorant(m, org, dir, __builtin_huge_val());
+    double max_maj = maxc(maj);
+    double u = pcg_double(st, inc);
+    int ch = min(max(int(u * 3), 0), 2);
+    for (int it = 0;; it++) {
+        double mc = comp(maj, ch);
+        if (mc <= 0) break;
+        if (it >= P.max_null) break;
+        double t = -log(1 - pcg_double(st, inc)) / mc;
+        double dt = t_hit - fl.accum_t;
+        fl.accum_t = fmin(fl.accum_t + t, t_hit);
+        if (t < dt) {
+            D3 sig_t = sigma_t_at(m, org + fl.accum_t * dir);
+            D3 real_prob = sig_t / maj;
+            D3 e = vexp(-1.0 * maj * t);
+            if (pcg_double(st, inc) < comp(real_prob, ch)) {
+                fl.scatter = true;
+                fl.T = fl.T * e / max_maj;
+                fl.trans_dir_pdf = fl.trans_dir_pdf * e * maj * real_prob / max_maj;
+                break;
+            }
+            D3 sig_n = maj - sig_t;
+            fl.T = fl.T * e * sig_n / max_maj;
+            fl.trans_dir_pdf = fl.trans_dir_pdf * e * maj * (1.0 - real_prob) / max_maj;
+        } else {
+            D3 e = vexp(-1.0 * maj * dt);
+            fl.T = fl.T * e;
+            fl.trans_dir_pdf = fl.trans_dir_pdf * e;
+            break;
+        }
+    }
+    return fl;
+}
+
+// Distance to the surface a ray hit (infinite on a miss), from the vertex
+// position the shading kernel computes.
+template <bool SPH>
+__device__ __forceinline__ double hit_distance(const DeviceScene &S, float4 h, D3 org, D3 dir) {
+    const int slot = __float_as_int(h.w);
+    if (slot < 0) return __builtin_huge_val();
+    Vertex v = make_vertex<SPH>(S, slot, h.x, h.y, h.z, org, dir);
+    return sqrt(dist2(org, v.pos));
+}
+
+// Free flight for every path of the volumetric queue that is in a medium;
+// lanes run their paths one after another (the loop lengths vary widely) in a
+// kernel small enough for full occupancy.
+template <bool SPH>
+__global__ __launch_bounds__(256) void k_vol_flight(DeviceScene S, PathBufs P) {
+    const int n = P.ctr[C_BUCKET + 0];
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t pid = P.bucket[i];
+        const int medium = P.medium[pid];
+        if (medium < 0) continue;
+        const float4 ro = P.ray_o[pid], rd = P.ray_d[pid];
+        const D3 org = f3d(ro), dir = f3d(rd);
+        uint64_t st = P.rng[pid];
+        const uint64_t inc = pcg_inc(slot_info(P, int(pid)).stream());
+        const Flight fl = free_flight(S, P, medium, org, dir, hit_distance<SPH>(S, P.hit[pid], org, dir), st, inc);
+        P.rng[pid] = st;
+        P.vf_t[pid] = make_double4(fl.T.x, fl.T.y, fl.T.z, fl.accum_t);
+        P.vf_p[pid] = make_double4(fl.trans_dir_pdf.x, fl.trans_dir_pdf.y, fl.trans_dir_pdf.z, fl.scatter ? 1.0 : 0.0);
+    }
+}
+
 template <bool SPH>
 __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &P, uint32_t pid, bool &push_ext,
                                           bool &push_sh, float4 &sh_o, float4 &sh_d, float4 &sh_c, float4 &sh_x) {
@@ -86,42 +167,24 @@ __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &
     bool alive = true;
     D3 p_scatter = org;
     if (medium >= 0) {
-        const MediumRec &m = S.media[medium];
-        D3 maj = medium_majorant(m, org, dir, __builtin_huge_val());
-        double max_maj = maxc(maj);
-        double u = pcg_double(st, inc);
-        int ch = min(max(int(u * 3), 0), 2);
-        double t_hit = hit ? sqrt(dist2(org, v.pos)) : __builtin_huge_val();
-        double accum_t = 0;
-        D3 T{1, 1, 1}, trans_dir_pdf{1, 1, 1};
-        for (int it = 0;; it++) {
-            double mc = comp(maj, ch);
-            if (mc <= 0) break;
-            if (it >= P.max_null) break;
-            double t = -log(1 - pcg_double(st, inc)) / mc;
-            double dt = t_hit - accum_t;
-            accum_t = fmin(accum_t + t, t_hit);
-            if (t < dt) {
-                D3 sig_t = sigma_t_at(m, org + accum_t * dir);
-                D3 real_prob = sig_t / maj;
-                D3 e = vexp(-1.0 * maj * t);
-                if (pcg_double(st, inc) < comp(real_prob, ch)) {
-                    scatter = true;
-                    never_scatter = false;
-                    T = T * e / max_maj;
-                    trans_dir_pdf = trans_dir_pdf * e * maj * real_prob / max_maj;
-                    break;
-                }
-                D3 sig_n = maj - sig_t;
-                T = T * e * sig_n / max_maj;
-                trans_dir_pdf = trans_dir_pdf * e * maj * (1.0 - real_prob) / max_maj;
-            } else {
-                D3 e = vexp(-1.0 * maj * dt);
-                T = T * e;
-                trans_dir_pdf = trans_dir_pdf * e;
-                break;
-            }
+        D3 T, trans_dir_pdf;
+        double accum_t;
+        if (P.vol_flight) {
+            // the free flight k_vol_flight sampled (it advanced the stream state)
+            const double4 ft = P.vf_t[pid], fp = P.vf_p[pid];
+            T = D3{ft.x, ft.y, ft.z};
+            trans_dir_pdf = D3{fp.x, fp.y, fp.z};
+            accum_t = ft.w;
+            scatter = fp.w != 0.0;
+        } else {
+            const Flight fl = free_flight(S, P, medium, org, dir, hit ? sqrt(dist2(org, v.pos)) : __builtin_huge_val(),
+                                          st, inc);
+            T = fl.T;
+            trans_dir_pdf = fl.trans_dir_pdf;
+            accum_t = fl.accum_t;
+            scatter = fl.scatter;
         }
+        if (scatter) never_scatter = false;
         if (isfinite(accum_t)) p_scatter = org + accum_t * dir;
         multi_trans_pdf = multi_trans_pdf * trans_dir_pdf;
         tp = tp * T / avgc(trans_dir_pdf);
@@ -414,6 +477,8 @@ void vol_alloc(bool, size_t capacity, PathBufs &P, Bufs &bufs, Alloc alloc) {
     P.nee_c = static_cast<float4 *>(take(capacity * sizeof(float4)));
     P.mtp = static_cast<float4 *>(take(capacity * sizeof(float4)));
     P.sh_x = static_cast<float4 *>(take(capacity * sizeof(float4)));
+    P.vf_t = static_cast<double4 *>(take(capacity * sizeof(double4)));
+    P.vf_p = static_cast<double4 *>(take(capacity * sizeof(double4)));
 }
 
 inline void vol_launch_check() {
@@ -422,6 +487,11 @@ inline void vol_launch_check() {
 }
 
 inline void vol_shade_all(const DeviceScene &S, PathBufs &P, int cur, int grid, hipStream_t s, bool sph) {
+    if (P.vol_flight) {
+        if (sph) hipLaunchKernelGGL(k_vol_flight<true>, dim3(grid), dim3(256), 0, s, S, P);
+        else hipLaunchKernelGGL(k_vol_flight<false>, dim3(grid), dim3(256), 0, s, S, P);
+        vol_launch_check();
+    }
     if (sph) hipLaunchKernelGGL(k_vol_shade<true>, dim3(grid), dim3(256), 0, s, S, P, cur);
     else hipLaunchKernelGGL(k_vol_shade<false>, dim3(grid), dim3(256), 0, s, S, P, cur);
     vol_launch_check();
