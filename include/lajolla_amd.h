@@ -343,7 +343,7 @@ typedef struct ljx_render_stats {
 typedef struct ljx_exec_options {
     int32_t collect_timing;        /* 1: HIP events around every kernel */
     int32_t collect_traversal;     /* 1: count node visits / tri tests */
-    int32_t max_paths_in_flight;   /* 0: library default (16,777,216, also the maximum) */
+    int32_t max_paths_in_flight;   /* 0: library default (67,108,864, also the maximum) */
     int32_t block_size;            /* 0: library default (multiple of 64) */
     int32_t sort_threshold;        /* 0: library default; <0: never sort */
     int32_t trace_blocks_per_cu;   /* traversal blocks per CU (0: library default, 8) */

@@ -104,9 +104,10 @@ struct PathBufs {
     long long work0;
 };
 
-// Shadow-queue entries carry the path slot in the low 24 bits and the vertex
-// index in the high 8.
-constexpr int kSlotBits = 24;
+// Shadow-queue entries carry the path slot in the low kSlotBits bits and the
+// vertex index in the rest (clamped there: only the guiding record, a few
+// vertices deep, reads it).
+constexpr int kSlotBits = 26;
 constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1;
 // Material types never guided: their lobes are near-specular transmission.
 __host__ __device__ constexpr bool guidable(int mt) { return mt != 2 && mt != 5; }
@@ -392,7 +393,7 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
                 sh_o = make_float4(float(v.pos.x), float(v.pos.y), float(v.pos.z), float(S.eps));
                 sh_d = make_float4(float(dir_light.x), float(dir_light.y), float(dir_light.z), float(tfar));
                 sh_c = make_float4(float(c.x), float(c.y), float(c.z),
-                                   __int_as_float(int(pid | (uint32_t(min(b, 255)) << kSlotBits))));
+                                   __int_as_float(int(pid | (uint32_t(min(b, (1 << (32 - kSlotBits)) - 1)) << kSlotBits))));
             }
         }
         // BSDF sampling (path_tracing.h:209-259)
