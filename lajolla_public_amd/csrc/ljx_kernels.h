@@ -4,14 +4,16 @@
 // into stages that each run over a compacted SoA queue of path ids:
 //
 //   k_raygen      sample_primary + the two screen-jitter draws
-//   k_extend      closest-hit traversal; appends each path to the bucket of
-//                 the hit material's type (material sort) or to MISS
+//   k_extend      closest-hit traversal, 64 rays per wave fetch; appends each
+//                 path to the bucket of the hit material's type (material
+//                 sort) or to MISS
 //   k_shade<MT>   one kernel per material type: emission + MIS of the hit
 //                 reached by the previous BSDF sample, Russian roulette,
 //                 throughput update, then NEE (light pick, point, shadow ray
 //                 with its pending contribution) and BSDF sampling
 //   k_miss        environment term of escaped paths
-//   k_shadow      any-hit traversal; adds the pending NEE contribution
+//   k_shadow      any-hit traversal; an occlusion byte per shadow ray
+//   k_shadow_resolve  adds the pending NEE contributions of unoccluded rays
 //   k_accumulate  per-pixel sum of the pass's samples in sample order
 //
 // Each path draws from its own PCG stream (stream = sample * W * H + pixel)

@@ -10,8 +10,10 @@
 // never_scatter (:505-557), Russian roulette from rr_depth (:331-400), and the
 // max_null_collisions cap (:760-764).
 //
-//   k_vol_shade   free flight, emission with MIS, index-matched pass-through,
-//                 phase or BSDF sampling, NEE light-point sampling, RR
+//   k_vol_flight  free flight (delta tracking) in scenes with grid media
+//   k_vol_shade   free flight in homogeneous media, emission with MIS,
+//                 index-matched pass-through, phase or BSDF sampling, NEE
+//                 light-point sampling, RR
 //   k_vol_shadow  the NEE shadow walk: closest hits through index-matched
 //                 surfaces with ratio tracking in every medium on the way
 //
