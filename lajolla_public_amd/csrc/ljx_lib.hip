@@ -1039,6 +1039,8 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
         if (gopt.train_spp_per_pass == 0) gopt.train_spp_per_pass = 1;
         int md = opt.max_depth > 0 ? opt.max_depth : 16;
         gopt.max_record_depth = std::max(1, std::min(gopt.max_record_depth > 0 ? gopt.max_record_depth : md, 16));
+        // shadow rays carry the index of their vertex clamped to the bits above the path slot
+        static_assert((1 << (32 - kSlotBits)) - 1 >= 16, "a recorded vertex index must fit beside the path slot");
     }
     ljx_exec_options e{};
     if (ex) e = *ex;
