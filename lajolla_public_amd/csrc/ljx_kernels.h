@@ -245,11 +245,18 @@ __global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int c
         if (i < n) {
             pid = q[i];
             const float4 o = P.ray_o[pid], d = P.ray_d[pid];
-            TraceHit h;
-            const bool hit = S.num_nodes > 0 && trace_ray<false, STATS, SPH>(ts, o.x, o.y, o.z, d.x, d.y, d.z, o.w, d.w,
-                                                                             stack + threadIdx.x, blockDim.x, spill,
-                                                                             sstride, h, nn, np);
+            Tracer<false, STATS, SPH> tr;
+            tr.start(o.x, o.y, o.z, d.x, d.y, d.z, o.w, d.w);
+            if (S.num_nodes > 0)
+                while (!tr.step(ts, stack + threadIdx.x, blockDim.x, spill, sstride, nn, np)) {
+                }
+            const bool hit = tr.found;
+            TraceHit h = tr.hit;
             if (hit) {
+                // the direction again for the FP64 distance (kept out of the
+                // traversal's registers when no spheres need it there)
+                const float4 d2 = SPH ? make_float4(tr.dx, tr.dy, tr.dz, 0.f) : P.ray_d[pid];
+                h.t = refine_t<SPH>(S.prims, h.slot, h.t, tr.r.ox, tr.r.oy, tr.r.oz, d2.x, d2.y, d2.z);
                 P.hit[pid] = make_float4(h.t, h.u, h.v, __int_as_float(h.slot));
                 const int sb = S.slot_bucket[h.slot];
                 // textured materials are shaded by the runtime-dispatch kernel
