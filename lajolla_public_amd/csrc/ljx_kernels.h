@@ -67,6 +67,8 @@ struct PathBufs {
     uint32_t *shq;
     float4 *sh_o, *sh_d, *sh_c;
     uint8_t *sh_occ;  // per shadow ray: 1 when occluded
+    uint8_t *alive;   // per path slot: 1 when shading gave it another ray (cleared as the queue is built)
+    int *alive_blk;   // per kScanBlock slots: live paths, then their exclusive prefix
     // volumetric path state
     int *medium;      // current medium id
     float4 *nee_c;    // nee_p_cache xyz, dir_pdf
@@ -159,6 +161,89 @@ __device__ __forceinline__ SlotInfo slot_info(const PathBufs &P, int i) {
 __global__ void k_reset_counters(int *ctr, int first, int count) {
     int i = threadIdx.x;
     if (i < count) ctr[first + i] = 0;
+}
+
+// The next bounce's extension queue: the slots shading marked alive, in slot
+// order (a stable compaction), so that neighbouring lanes of the next
+// bounce's kernels read neighbouring path state and trace rays from
+// neighbouring pixels.  Three launches over the pass's n slots: counts per
+// kScanBlock slots, their exclusive scan (which also sets the queue's
+// length), and the scatter, which clears the flags it read.
+constexpr int kScanBlock = 4096;  // 256 threads x 16 slots
+__device__ __forceinline__ int alive_in(uint4 f) {
+    // flags are 0 or 1 bytes
+    return __popc(f.x) + __popc(f.y) + __popc(f.z) + __popc(f.w);
+}
+__device__ __forceinline__ int block_excl_scan(int v, int &total) {
+    __shared__ int wsum[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+    for (int k = 0; k < 4; k++) {
+        if (k < w) before += wsum[k];
+        total += wsum[k];
+    }
+    __syncthreads();
+    return before + x - v;
+}
+// Clears the flags of a pass's n slots (rounded up to 16); the scatter keeps
+// them clear from then on.
+__global__ __launch_bounds__(256) void k_alive_clear(PathBufs P, int n) {
+    const int n16 = (n + 15) / 16;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += gridDim.x * blockDim.x)
+        reinterpret_cast<uint4 *>(P.alive)[i] = make_uint4(0, 0, 0, 0);
+}
+__global__ __launch_bounds__(256) void k_alive_count(PathBufs P, int n) {
+    const int s0 = blockIdx.x * kScanBlock + threadIdx.x * 16;
+    const int c = s0 < n ? alive_in(*reinterpret_cast<const uint4 *>(P.alive + s0)) : 0;
+    int total;
+    block_excl_scan(c, total);
+    if (threadIdx.x == 0) P.alive_blk[blockIdx.x] = total;
+}
+__global__ __launch_bounds__(1024) void k_alive_scan(PathBufs P, int nblk, int cur) {
+    // 1024 threads x 16 blocks each: nblk <= 16384 (n <= 2^26)
+    __shared__ int part[1024];
+    const int b0 = threadIdx.x * 16;
+    int v[16], sum = 0;
+    for (int k = 0; k < 16; k++) {
+        v[k] = b0 + k < nblk ? P.alive_blk[b0 + k] : 0;
+        sum += v[k];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int y = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+        __syncthreads();
+        part[threadIdx.x] += y;
+        __syncthreads();
+    }
+    int run = part[threadIdx.x] - sum;
+    for (int k = 0; k < 16; k++) {
+        if (b0 + k < nblk) P.alive_blk[b0 + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 1023) P.ctr[C_EXT + (cur ^ 1)] = part[1023];
+}
+__global__ __launch_bounds__(256) void k_alive_scatter(PathBufs P, int n, int cur) {
+    uint32_t *next = cur ? P.extq0 : P.extq1;
+    const int s0 = blockIdx.x * kScanBlock + threadIdx.x * 16;
+    uint4 f = make_uint4(0, 0, 0, 0);
+    if (s0 < n) f = *reinterpret_cast<const uint4 *>(P.alive + s0);
+    int total;
+    int k = P.alive_blk[blockIdx.x] + block_excl_scan(alive_in(f), total);
+    if (s0 < n && (f.x | f.y | f.z | f.w)) {
+        const uint32_t w[4] = {f.x, f.y, f.z, f.w};
+        for (int j = 0; j < 16; j++)
+            if ((w[j >> 2] >> (8 * (j & 3))) & 0xffu) next[k++] = uint32_t(s0 + j);
+        *reinterpret_cast<uint4 *>(P.alive + s0) = make_uint4(0, 0, 0, 0);
+    }
 }
 
 __global__ void k_fill_f64(double *p, size_t n) {
@@ -456,7 +541,6 @@ __device__ __forceinline__ void shade_queue(const DeviceScene &S, const PathBufs
     constexpr int B = MT >= 0 ? MT : kBucketAny;
     int n = P.ctr[C_BUCKET + B];
     const uint32_t *q = P.bucket + size_t(B) * P.capacity;
-    uint32_t *next = cur ? P.extq0 : P.extq1;
     int stride = gridDim.x * blockDim.x;
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         int i = base + threadIdx.x;
@@ -467,8 +551,7 @@ __device__ __forceinline__ void shade_queue(const DeviceScene &S, const PathBufs
             pid = q[i];
             shade_surface<MT, SPH, GUIDE>(S, P, pid, pe, ps, so, sd, sc);
         }
-        int k = wave_append(pe, &P.ctr[C_EXT + (cur ^ 1)]);
-        if (k >= 0) next[k] = pid;
+        if (pe) P.alive[pid] = 1;
         int j = wave_append(ps, &P.ctr[C_SH]);
         if (j >= 0) {
             P.shq[j] = pid;

@@ -877,6 +877,9 @@ void ensure_ctx(Scene &sc, int capacity, int trace_grid) {
     P.sh_d = sc.alloc<float4>(C, rc->bufs);
     P.sh_c = sc.alloc<float4>(C, rc->bufs);
     P.sh_occ = sc.alloc<uint8_t>(C, rc->bufs);
+    const size_t nblk = (C + kScanBlock - 1) / kScanBlock;
+    P.alive = sc.alloc<uint8_t>(nblk * kScanBlock, rc->bufs);
+    P.alive_blk = sc.alloc<int>(nblk, rc->bufs);
     P.ctr = sc.alloc<int>(C_TOTAL, rc->bufs);
     P.stat = sc.alloc<unsigned long long>(S_NUM, rc->bufs);
     P.spill = sc.alloc<int>(size_t(rc->trace_grid) * 256 * sc.spill_entries, rc->bufs);
@@ -938,6 +941,15 @@ void trace_stage_t(Scene &sc, PathBufs &P, int cur, int grid, int sgrid, hipStre
 // and 723 Msamples/s with 4M, 16M, 32M and 64M paths per pass.  The path
 // state is ~0.5 KB per path (31 GB for a 1080p pass of 32 samples per pixel).
 constexpr int kDefaultPathsInFlight = 1 << kSlotBits;
+
+// The next extension queue from the flags shading left (k_alive_*).
+void build_ext_queue(PathBufs &P, int n, int cur, hipStream_t s) {
+    const int nblk = (n + kScanBlock - 1) / kScanBlock;
+    if (nblk > 16384) fail(LJX_ERR_UNSUPPORTED, "pass larger than the queue builder handles");
+    launch(k_alive_count, nblk, 256, s, P, n);
+    launch(k_alive_scan, 1, 1024, s, P, nblk, cur);
+    launch(k_alive_scatter, nblk, 256, s, P, n, cur);
+}
 
 void trace_stage(Scene &sc, PathBufs &P, int cur, int grid, int sgrid, hipStream_t s, bool stats, bool shadow) {
     if (stats) {
@@ -1137,6 +1149,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
             tm.begin(3, ev);
             launch(k_reset_counters, 1, 64, s, P.ctr, 0, C_TOTAL);
             launch(k_raygen, shade_grid, 256, s, S, P, (W % 8 || H % 8) ? 1 : 0);
+            launch(k_alive_clear, shade_grid, 256, s, P, ns * nq);
             if (vol) launch(k_vol_init, shade_grid, 256, s, S, P);
             if (recording)
                 launch(k_guide_clear, shade_grid, 256, s, P.rec_dir, P.rec_beta, P.rec_c,
@@ -1161,6 +1174,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
                 tm.begin(2, ev);
                 if (vol) vol_shade_all(S, P, cur, shade_grid, s, sc.has_spheres);
                 else shade_all(sc, P, cur, shade_grid, s, guided, shade_block);
+                build_ext_queue(P, ns * nq, cur, s);
                 tm.end(2, ev);
                 tm.begin(1, ev);
                 if (vol) vol_shadow(S, P, tgrid, s, stats_on, sc.has_spheres);
@@ -1455,12 +1469,14 @@ void batch_impl(const ljx_batch_job *jobs, size_t njobs, const ljx_exec_options 
                 passes++;
                 launch(k_reset_counters, 1, 64, cs, P.ctr, 0, C_TOTAL);
                 launch(k_raygen, sgrid, 256, cs, S, P, 1);
+                launch(k_alive_clear, sgrid, 256, cs, P, n);
                 if (vol) launch(k_vol_init, sgrid, 256, cs, S, P);
                 int cur = 0;
                 for (int b = 0; b < std::max(P.max_depth, 1); b++) {
                     trace_stage(sc, P, cur, tgrid, sgrid, cs, false, false);
                     if (vol) vol_shade_all(S, P, cur, sgrid, cs, sc.has_spheres);
                     else shade_all(sc, P, cur, sgrid, cs, false, gp.block);
+                    build_ext_queue(P, n, cur, cs);
                     if (vol) vol_shadow(S, P, tgrid, cs, false, sc.has_spheres);
                     else trace_stage(sc, P, cur, tgrid, sgrid, cs, false, true);
                     cur ^= 1;

@@ -326,7 +326,6 @@ template <bool SPH>
 __global__ __launch_bounds__(256) void k_vol_shade(DeviceScene S, PathBufs P, int cur) {
     int n = P.ctr[C_BUCKET + 0];
     const uint32_t *q = P.bucket;
-    uint32_t *next = cur ? P.extq0 : P.extq1;
     int stride = gridDim.x * blockDim.x;
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         int i = base + threadIdx.x;
@@ -337,8 +336,7 @@ __global__ __launch_bounds__(256) void k_vol_shade(DeviceScene S, PathBufs P, in
             pid = q[i];
             vol_shade<SPH>(S, P, pid, pe, ps, so, sd, sc, sx);
         }
-        int k = wave_append(pe, &P.ctr[C_EXT + (cur ^ 1)]);
-        if (k >= 0) next[k] = pid;
+        if (pe) P.alive[pid] = 1;
         int j = wave_append(ps, &P.ctr[C_SH]);
         if (j >= 0) {
             P.shq[j] = pid;
