@@ -187,6 +187,11 @@ def main():
             "prims_per_ray": round(tris_per_ray, 2),
             "launches_per_step": ext_launches // args.steps,
             "avg_launch_ms": round(ext_ms / max(1, ext_launches), 4),
+            "hbm_traffic_gbs": round(traffic / (ext_ms / max(1, ext_launches)) / 1e6, 1) if traffic else None,
+            "note": "algorithmic bytes count every node and primitive fetch of every ray as HBM traffic; the "
+                    "caches serve most of them (traffic_over_algorithmic), so achieved can exceed the HBM peak. "
+                    "The kernel is bound by the vector-memory pipeline and divergence "
+                    "(profiles/pmc_pipeline_terrain_spheres.md), not by HBM bandwidth (hbm_traffic_gbs).",
         },
         "cpu_baseline": cpu,
         "guiding": guiding,
