@@ -293,8 +293,8 @@ __global__ void k_raygen(DeviceScene S, PathBufs P, int compact) {
 
 // Closest-hit traversal over the extension queue, then material bucketing.
 // Closest-hit traversal of the extension queue.  A wave takes 64 rays at a
-// time with one atomic, so waves that drew short rays take more (a launch
-// has only a few rays per lane); each hit's distance is refined while its
+// time with one atomic, so waves that drew short rays take more and the
+// launch's tail stays short; each hit's distance is refined while its
 // triangle is in cache, and the wave appends its paths to the buckets of
 // the materials they hit (the material sort) with one atomic per bucket.
 // The fetch counter of a launch starts at zero: k_reset_counters clears
@@ -307,7 +307,6 @@ __global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int c
     const int n = P.ctr[C_EXT + cur];
     const bool sorted = n >= P.sort_threshold;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        P.ctr[C_EXT + (cur ^ 1)] = 0;
         P.ctr[C_SH] = 0;
         P.ctr[C_XFETCH + (cur ^ 1)] = 0;
         atomicAdd(&P.stat[S_EXT_RAYS], (unsigned long long)n);
