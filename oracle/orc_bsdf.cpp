@@ -204,7 +204,7 @@ inline void disney_alphas(Real roughness, Real anisotropic, Real &ax, Real &ay) 
 GlassLobe disney_glass_lobe(V3 base, Real roughness, Real anisotropic, Real eta) {
     GlassLobe g;
     g.Kr = base;
-    g.Kt = vsqrt0(base);
+    g.Kt = planted(7) ? base : vsqrt0(base);
     g.eta_param = eta;
     g.aniso = true;
     g.rough = roughness;
@@ -223,14 +223,14 @@ V3 disney_diffuse_eval(V3 base, Real roughness, Real subsurface, V3 in, V3 out, 
     V3 h = normalize(in + out);
     Real n_in = std::fabs(dot(f.n, in)), n_out = std::fabs(dot(f.n, out));
     Real h_out = std::fabs(dot(h, out));
-    Real FD90 = Real(0.5) + 2 * roughness * h_out * h_out;
+    Real FD90 = Real(0.5) + (planted(3) ? 1 : 2) * roughness * h_out * h_out;
     Real FD_in = 1 + (FD90 - 1) * pow5(1 - n_in);
     Real FD_out = 1 + (FD90 - 1) * pow5(1 - n_out);
     V3 base_diffuse = base / kPi * (FD_in * FD_out * n_out);
     Real FSS90 = roughness * h_out * h_out;
     Real FSS_in = 1 + (FSS90 - 1) * pow5(1 - n_in);
     Real FSS_out = 1 + (FSS90 - 1) * pow5(1 - n_out);
-    V3 ss = Real(1.25) * base / kPi *
+    V3 ss = (planted(6) ? Real(1) : Real(1.25)) * base / kPi *
             (FSS_in * FSS_out * (1 / (n_in + n_out) - Real(0.5)) + Real(0.5)) * n_out;
     return (1 - subsurface) * base_diffuse + subsurface * ss;
 }
@@ -312,7 +312,9 @@ V3 disney_sheen_eval(V3 base, Real sheen_tint, V3 in, V3 out, const ShadePoint &
     Frame f = front_frame(v, in);
     V3 h = normalize(in + out);
     V3 C = (1 - sheen_tint) + sheen_tint * tint_color(base);
-    return C * (pow5(1 - std::fabs(dot(h, out))) * std::fabs(dot(f.n, out)));
+    Real s5 = pow5(1 - std::fabs(dot(h, out)));
+    if (planted(1)) s5 *= 1 - std::fabs(dot(h, out));
+    return C * (s5 * std::fabs(dot(f.n, out)));
 }
 
 // homework1.tex:460-548 -- the combined BSDF and its lobe weights.
@@ -322,16 +324,16 @@ struct DisneyWeights {
 DisneyWeights disney_weights(const Material &m) {
     DisneyWeights w;
     w.diffuse = (1 - m.specular_transmission) * (1 - m.metallic);
-    w.sheen = (1 - m.metallic) * m.sheen;
+    w.sheen = (planted(2) ? 1 : (1 - m.metallic)) * m.sheen;
     w.metal = (1 - m.specular_transmission * (1 - m.metallic));
-    w.clearcoat = Real(0.25) * m.clearcoat;
+    w.clearcoat = (planted(4) ? Real(0.5) : Real(0.25)) * m.clearcoat;
     w.glass = (1 - m.metallic) * m.specular_transmission;
     return w;
 }
 V3 disney_metal_F0(const Material &m) {
     Real R0 = (m.eta - 1) * (m.eta - 1) / ((m.eta + 1) * (m.eta + 1));
     V3 Ks = (1 - m.specular_tint) + m.specular_tint * tint_color(m.a);
-    return m.specular * R0 * (1 - m.metallic) * Ks + m.metallic * m.a;
+    return m.specular * R0 * (planted(9) ? 1 : (1 - m.metallic)) * Ks + m.metallic * m.a;
 }
 V3 disney_bsdf_eval(const Material &m, V3 in, V3 out, const ShadePoint &v, bool to_view) {
     Real ax, ay;
@@ -354,6 +356,7 @@ void disney_probs(const Material &m, bool inside, Real p[4]) {
         p[2] = 1;
         return;
     }
+    if (planted(8)) w.clearcoat *= 2;
     Real tot = w.diffuse + w.metal + w.glass + w.clearcoat;
     p[0] = w.diffuse / tot;
     p[1] = w.metal / tot;

@@ -25,6 +25,12 @@ constexpr Real kInvTwoPi = 1.0 / kTwoPi;
 constexpr Real kInvFourPi = 1.0 / (4.0 * kPi);
 inline Real inf() { return std::numeric_limits<Real>::infinity(); }
 
+// Planted mistakes, for tests that show their checks can fail: 0 (the
+// default) is the restatement as written; k > 0 swaps in one wrong formula
+// (the list is at orc_set_plant in orc_render.cpp).  Never set outside tests.
+inline int g_plant = 0;
+inline bool planted(int k) { return g_plant == k; }
+
 // ---------------------------------------------------------------- vectors
 struct V3 {
     Real x = 0, y = 0, z = 0;

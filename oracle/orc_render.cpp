@@ -829,6 +829,7 @@ static V3 path_tracing(const Scene &s, int x, int y, Pcg &rng, Counters &cnt) {
             PointAndNormal lp{bv.position, bv.geometric_normal};
             Real p1 = pmf(s.light_dist, lid) * pdf_point_on_light(s, s.lights[lid], lp, vertex.position);
             Real w2 = (p2 * p2) / (p1 * p1 + p2 * p2);
+            if (planted(5)) w2 *= 2;
             C2 /= p2;
             radiance += throughput * C2 * w2;
         } else if (!hit && s.envmap_light_id >= 0) {
@@ -1370,6 +1371,22 @@ static void parallel_for(size_t n, int threads, F fn) {
 using namespace orc;
 
 extern "C" {
+
+// Selects a planted mistake (tests only; 0 restores the restatement):
+//   1 sheen: (1 - |h.out|)^6 in place of ^5           (homework1.tex:412)
+//   2 DisneyBSDF sheen weight without (1 - metallic)  (homework1.tex:466)
+//   3 diffuse F_D90 = 0.5 + r |h.out|^2 (factor 2 dropped)  (homework1.tex:107)
+//   4 DisneyBSDF clearcoat weight 0.5 in place of 0.25      (homework1.tex:468)
+//   5 path_tracing: MIS weight of BSDF-sampled area-light hits doubled
+//     (path_tracing.h:280-298)
+//   6 subsurface factor 1.0 in place of 1.25          (homework1.tex:122)
+//   7 DisneyGlass transmission with baseColor, not its square root
+//   8 DisneyBSDF clearcoat sampling probability doubled (homework1.tex:538)
+//   9 DisneyBSDF specular C_0 without (1 - metallic)   (homework1.tex:481)
+int orc_set_plant(int k) {
+    g_plant = k;
+    return LJX_OK;
+}
 
 int orc_last_error(char *buf, size_t cap) {
     if (!buf || cap == 0) return LJX_ERR_INVALID_ARGUMENT;

@@ -63,6 +63,7 @@ def lib():
         L.orc_guide_loss_grad.argtypes = [dp, fp, fp, C.c_int, C.c_int, dp]
         L.orc_guide_loss_grad.restype = C.c_double
         L.orc_guide_features.argtypes = [dp, dp, dp, dp, dp, fp]
+        L.orc_set_plant.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -182,6 +183,23 @@ class OracleScene:
         out = np.zeros(len(self.desc.lights))
         lib().orc_light_pmf(self.h, _dp(out))
         return out
+
+
+class planted:
+    """Context manager: the oracle runs with planted mistake k (the list is
+    at orc_set_plant in orc_render.cpp) and is restored on exit.  Tests use
+    it to show that a check fails on a plausible mistake."""
+
+    def __init__(self, k):
+        self.k = k
+
+    def __enter__(self):
+        lib().orc_set_plant(self.k)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_plant(0)
+        return False
 
 
 def pcg_u32(stream, seed, n):
