@@ -133,10 +133,13 @@ def main():
     node_bytes = inst["node_bytes"]
     bytes_per_ray = 48 + node_bytes * nodes_per_ray + 36 * tris_per_ray
     alg_bytes = ext_rays * bytes_per_ray
-    achieved = alg_bytes / (ext_ms / 1000) / 1e9 if ext_ms > 0 else 0.0
+    achieved_alg = alg_bytes / (ext_ms / 1000) / 1e9 if ext_ms > 0 else 0.0
     kernel_ms = {k: sum(s[k] for s in stats) / args.steps for k in ("extend_ms", "shadow_ms", "shade_ms", "other_ms")}
     alg_per_launch = alg_bytes / max(1, ext_launches)
+    avg_launch_ms = ext_ms / max(1, ext_launches)
     traffic = measured_traffic(args.workload)
+    hbm_gbs = traffic / (avg_launch_ms / 1000) / 1e9 if traffic and avg_launch_ms > 0 else None
+    pipeline = measured_pipeline(args.workload)
 
     cpu = None
     if not args.no_cpu:
@@ -160,7 +163,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 traversal / f64 shading",
+        "dtype": "f32 traversal and path state between kernels (throughput, radiance); f64 shading and per-pixel sums",
         "data": "synthetic (seeded scene generator)",
         "config": {
             "workload": workload,
@@ -173,25 +176,27 @@ def main():
         "roofline": {
             "kernel": "k_extend (closest-hit traversal)",
             "bound": "hbm",
-            "achieved": round(achieved, 1),
+            "achieved": round(hbm_gbs, 1) if hbm_gbs else None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac": round(hbm_gbs / HBM_PEAK_GBS, 4) if hbm_gbs else None,
             "traffic": traffic,
-            "traffic_source": "profiles/pmc_k_extend_terrain_spheres.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE "
-                              "x1.0, scaled by a copy kernel of known size)" if traffic else None,
+            "traffic_source": f"profiles/pmc_k_extend_{args.workload}.md (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE "
+                              "x1.0 per dispatch, scaled by a copy kernel of known size)" if traffic else None,
+            "avg_launch_ms": round(avg_launch_ms, 4),
+            "launches_per_step": ext_launches // args.steps,
+            "algorithmic_gbs": round(achieved_alg, 1),
             "algorithmic_bytes_per_launch": round(alg_per_launch),
             "traffic_over_algorithmic": round(traffic / alg_per_launch, 3) if traffic else None,
             "bytes_per_ray": round(bytes_per_ray, 1),
             "nodes_per_ray": round(nodes_per_ray, 2),
             "prims_per_ray": round(tris_per_ray, 2),
-            "launches_per_step": ext_launches // args.steps,
-            "avg_launch_ms": round(ext_ms / max(1, ext_launches), 4),
-            "hbm_traffic_gbs": round(traffic / (ext_ms / max(1, ext_launches)) / 1e6, 1) if traffic else None,
-            "note": "algorithmic bytes count every node and primitive fetch of every ray as HBM traffic; the "
-                    "caches serve most of them (traffic_over_algorithmic), so achieved can exceed the HBM peak. "
-                    "The kernel is bound by the vector-memory pipeline and divergence "
-                    "(profiles/pmc_pipeline_terrain_spheres.md), not by HBM bandwidth (hbm_traffic_gbs).",
+            "pipeline": pipeline,
+            "note": "achieved and frac are HBM bytes measured by counters (traffic, per launch) over the live "
+                    "HIP-event launch time, against the 8 TB/s HBM peak. algorithmic_gbs counts every node and "
+                    "primitive fetch of every ray (48 B + node_bytes x nodes + 36 B x primitives per ray); the "
+                    "caches serve most of those, so it is not an HBM figure. The unit that bounds the kernel is "
+                    "the vector-memory pipeline (pipeline: texture address / data units busy fractions).",
         },
         "cpu_baseline": cpu,
         "guiding": guiding,
@@ -414,6 +419,20 @@ def guided_vs_plain(ref_spp, plain_spp=64, train_spp=8, steps_per_pass=6, plain_
     }
 
 
+def measured_pipeline(workload):
+    """Busy fractions of the vector-memory pipeline in k_extend from the
+    committed rocprofv3 PMC summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_pipeline_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    out = {}
+    for line in open(path):
+        if line.startswith("busy k_extend "):
+            k, v = line[len("busy k_extend "):].split(":")
+            out[k.strip()] = float(v)
+    return out or None
+
+
 def measured_traffic(workload):
     """HBM bytes per k_extend launch from the committed rocprofv3 PMC summary
     of this workload, or None if there is none."""
@@ -426,13 +445,30 @@ def measured_traffic(workload):
     return None
 
 
+def host_cpus():
+    """CPUs this process may run on: the affinity mask, capped by a cgroup v2
+    CPU quota where one is set; and nproc as the OS reports it."""
+    usable = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    if quota:
+        usable = max(1, min(usable, int(quota)))
+    return usable, os.cpu_count(), quota
+
+
 def cpu_baseline(desc, cpu_spp):
     """The oracle (lajolla's path_render / path_tracing restated in FP64 with
-    the reference's tile RNG plan, CPU SAH BVH in place of Embree) on the
-    host cores, over every tile of the same image at reduced spp."""
+    the reference's tile RNG plan, CPU SAH BVH in place of Embree) on every
+    host CPU this process may use, over every tile of the same image at
+    reduced spp."""
     from lajolla_public_amd import scenes
     from oracle import pyoracle as po
-    threads = min(os.cpu_count() or 1, 16)
+    threads, nproc, quota = host_cpus()
     t0 = time.time()
     orc = po.OracleScene(desc)
     build_s = time.time() - t0
@@ -450,6 +486,8 @@ def cpu_baseline(desc, cpu_spp):
         "value": round(cnt["paths"] / dt / 1e6, 4),
         "unit": "Msamples/s",
         "cores": threads,
+        "nproc": nproc,
+        "cgroup_cpu_quota": quota,
         "kind": "port",
         "sample": f"every 16x16 tile of the {desc.camera['width']}x{desc.camera['height']} image at {spp} spp "
                   f"({cnt['paths']} camera paths, "
