@@ -421,6 +421,19 @@ int ljx_trace_any_device(void *scene, const ljx_ray *rays, size_t n,
                          uint8_t *occluded, const ljx_exec_options *exec,
                          double *elapsed_ms, ljx_render_stats *stats);
 
+/* Closest hit like ljx_trace_closest_device (device pointers, identical
+ * hits), for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): it
+ * calls no allocator.  It has no counterpart in the reference.  Its only
+ * device memory beyond rays and hits is a fixed 16 MiB static array per
+ * device (the traversal stack spill of its grid), held until the process
+ * ends; a scene whose traversal would need more than that per block is
+ * rejected with LJX_ERR_UNSUPPORTED.  No timing or traversal counters.
+ * Blocks until the hits are written.  Calls that overlap on one device share
+ * that array and race: serialise them. */
+int ljx_trace_closest_device_noalloc(void *scene, const ljx_ray *rays, size_t n,
+                                     ljx_hit *hits);
+
 /* Evaluates eval / pdf_sample_bsdf / sample_bsdf of the scene's materials on
  * the device for n queries in host memory, with the same device code the
  * shading kernels run.  Replaces the three dispatchers in

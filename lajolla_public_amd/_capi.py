@@ -333,6 +333,7 @@ SIGNATURES = {
     "ljx_trace_any": ([VP, C.POINTER(Ray), C.c_size_t, C.POINTER(C.c_uint8)], C.c_int),
     "ljx_trace_closest_device": ([VP, VP, C.c_size_t, VP, C.POINTER(ExecOptions),
                                   C.POINTER(C.c_double), C.POINTER(RenderStats)], C.c_int),
+    "ljx_trace_closest_device_noalloc": ([VP, VP, C.c_size_t, VP], C.c_int),
     "ljx_trace_any_device": ([VP, VP, C.c_size_t, VP, C.POINTER(ExecOptions),
                               C.POINTER(C.c_double), C.POINTER(RenderStats)], C.c_int),
     "ljx_bsdf_batch": ([VP, C.POINTER(BSDFQuery), C.c_size_t, C.POINTER(BSDFResult)], C.c_int),

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -818,6 +819,34 @@ struct SpillBuf {
         if (p) hipFree(p);
     }
 };
+
+// ---------------------------------------------- allocation-free closest hit
+// Device memory of ljx_trace_closest_device_noalloc: the per-lane traversal
+// stack spill of its persistent grid (256 lanes x the scene's spill entries
+// per block), 16 MiB.  Each device holds its own copy from module load until
+// the process ends.  Nothing in it needs clearing: a lane writes a spill
+// entry before it reads it.  Calls that overlap on one device share it and
+// race.
+constexpr size_t kScratchInts = size_t(4) << 20;
+static __device__ __align__(256) int scratch[kScratchInts];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the symbol's host-side name is not a device address).
+int *scratch_on(int device) {
+    if (device < 0 || device >= kMaxDevices) fail(LJX_ERR_INVALID_ARGUMENT, "device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        HIPX(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) fail(LJX_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[device] = static_cast<int *>(p);
+    }
+    return g_scratch_ptr[device];
+}
 
 // ------------------------------------------------------------ render loop
 struct Timer {
@@ -1706,6 +1735,28 @@ int ljx_trace_closest_device(void *scene, const ljx_ray *rays, size_t n, ljx_hit
         Scene *sc = as_scene(scene);
         if (n && (!rays || !hits)) fail(LJX_ERR_INVALID_ARGUMENT, "NULL buffer");
         trace_device<false>(*sc, rays, n, hits, exec, elapsed_ms, stats);
+    });
+}
+
+int ljx_trace_closest_device_noalloc(void *scene, const ljx_ray *rays, size_t n, ljx_hit *hits) {
+    return guarded([&]() {
+        Scene *sc = as_scene(scene);
+        if (n && (!rays || !hits)) fail(LJX_ERR_INVALID_ARGUMENT, "NULL buffer");
+        HIPX(hipSetDevice(sc->device));
+        int *spill = scratch_on(sc->device);
+        if (n == 0) return;
+        const size_t per_block = size_t(256) * size_t(sc->spill_entries);
+        if (per_block > kScratchInts)
+            fail(LJX_ERR_UNSUPPORTED, "the scene's traversal needs " + std::to_string(per_block) +
+                                          " stack-spill entries per block, more than the " +
+                                          std::to_string(kScratchInts) + " of the allocation-free scratch");
+        // the same persistent grid as ljx_trace_closest_device, capped by what
+        // the scratch holds; rays do not depend on the grid
+        const int grid = int(std::min<size_t>(size_t(trace_grid_for(*sc)), kScratchInts / per_block));
+        hipLaunchKernelGGL(k_trace_batch<false>, dim3(grid), dim3(256), 0, sc->stream, sc->ds, rays, n, hits,
+                           nullptr, spill, nullptr, 0);
+        HIPX(hipGetLastError());
+        HIPX(hipStreamSynchronize(sc->stream));
     });
 }
 
