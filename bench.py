@@ -91,6 +91,7 @@ def main():
     print(f"[bench] scene generated in {time.time() - t0:.1f}s: {desc.num_triangles} triangles", file=sys.stderr,
           flush=True)
     sc = api.Scene(desc)
+    bvh = sc.bvh_info()
     W, H, spp = desc.camera["width"], desc.camera["height"], desc.options["samples_per_pixel"]
     lib = sc.lib
     out = api.DeviceBuffer(W * H * 3 * 4)
@@ -171,6 +172,12 @@ def main():
             "triangles": desc.num_triangles, "seed": desc.meta["seed"],
         },
         "grays_per_s": round(grays, 4),
+        "scene_build": {"builder": "device" if bvh["builder"] == capi.BVH_BUILDER_GPU else "host",
+                        "scene_build_ms": round(bvh["build_ms"], 2), "bvh_build_ms": round(bvh["bvh_ms"], 2),
+                        "bvh_nodes": int(bvh["num_nodes"]), "bvh_depth": int(bvh["depth"]),
+                        "sah_cost": round(bvh["sah_cost"], 4),
+                        "note": "first scene creation in the process (includes one-time kernel loading); not in "
+                                "the timed region"},
         "rays_per_step": {"extension": ext_rays // args.steps, "shadow": sh_rays // args.steps},
         "kernel_ms_per_step": {k: round(v, 2) for k, v in kernel_ms.items()},
         "roofline": {

@@ -372,10 +372,41 @@ int ljx_scene_desc_free(ljx_scene_desc *desc);
  * (src/parsers/parse_scene.h:9). */
 int ljx_scene_load(const char *xml_path, int device, void **scene_out);
 
-/* Builds the device scene: uploads geometry, builds the SAH BVH, light and
- * triangle sampling tables and the scene bounding sphere.  Replaces
- * Scene::Scene (src/scene.cpp:4-53) including rtcCommitScene. */
+/* Builds the device scene: uploads geometry, builds the SAH BVH on the
+ * device, light and triangle sampling tables and the scene bounding sphere.
+ * Replaces Scene::Scene (src/scene.cpp:4-53) including rtcCommitScene. */
 int ljx_scene_create(const ljx_scene_desc *desc, int device, void **scene_out);
+
+/* BVH build choices (rtcSetSceneBuildQuality, src/scene.cpp:20-27, is the
+ * reference's one choice).  Both builders give the same tree. */
+enum { LJX_BVH_BUILDER_GPU = 0, LJX_BVH_BUILDER_HOST = 1 };
+typedef struct ljx_build_options {
+    int32_t bvh_builder;           /* LJX_BVH_BUILDER_* (default GPU) */
+    int32_t max_leaf_size;         /* primitives per leaf at most, 1..8 (0: 8) */
+} ljx_build_options;
+
+/* ljx_scene_create with build options (NULL: the defaults). */
+int ljx_scene_create_ex(const ljx_scene_desc *desc, int device, const ljx_build_options *build,
+                        void **scene_out);
+
+/* The scene's BVH: sizes, quality and a fingerprint.  No reference
+ * counterpart (Embree's tree is opaque). */
+typedef struct ljx_bvh_info {
+    int32_t builder;               /* LJX_BVH_BUILDER_* that built it */
+    int32_t depth;                 /* 4-wide nodes on the longest root-to-leaf path */
+    int32_t stack_need;            /* most traversal stack entries a ray can hold */
+    int32_t binary_depth;          /* depth of the binary SAH tree before the collapse */
+    uint64_t num_nodes;            /* 4-wide nodes */
+    uint64_t num_slots;            /* leaf slots (= primitives) */
+    uint64_t node_hash;            /* FNV-1a (64-bit) of the node bytes */
+    uint64_t slot_hash;            /* FNV-1a (64-bit) of the slot-to-primitive table (int32) */
+    double sah_cost;               /* expected node visits + primitive tests of a ray through the root box */
+    double bvh_ms;                 /* wall time of the BVH build (boxes, tree, slot records) */
+    double build_ms;               /* wall time of the whole scene build */
+} ljx_bvh_info;
+
+/* Fills info; downloads the nodes, so it is for tests and tools, not loops. */
+int ljx_scene_bvh_info(void *scene, ljx_bvh_info *info);
 
 /* Frees every device buffer of the scene.  Replaces Scene::~Scene
  * (src/scene.cpp:55-59). */

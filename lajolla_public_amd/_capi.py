@@ -300,6 +300,29 @@ class BatchJob(C.Structure):
     ]
 
 
+class BuildOptions(C.Structure):
+    _fields_ = [("bvh_builder", C.c_int32), ("max_leaf_size", C.c_int32)]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [
+        ("builder", C.c_int32),
+        ("depth", C.c_int32),
+        ("stack_need", C.c_int32),
+        ("binary_depth", C.c_int32),
+        ("num_nodes", C.c_uint64),
+        ("num_slots", C.c_uint64),
+        ("node_hash", C.c_uint64),
+        ("slot_hash", C.c_uint64),
+        ("sah_cost", C.c_double),
+        ("bvh_ms", C.c_double),
+        ("build_ms", C.c_double),
+    ]
+
+
+BVH_BUILDER_GPU, BVH_BUILDER_HOST = 0, 1
+
+
 class BatchStats(C.Structure):
     _fields_ = [
         ("total_ms", C.c_double),
@@ -323,6 +346,8 @@ SIGNATURES = {
     "ljx_scene_desc_free": ([C.POINTER(SceneDescC)], C.c_int),
     "ljx_scene_load": ([C.c_char_p, C.c_int, C.POINTER(VP)], C.c_int),
     "ljx_scene_create": ([C.POINTER(SceneDescC), C.c_int, C.POINTER(VP)], C.c_int),
+    "ljx_scene_create_ex": ([C.POINTER(SceneDescC), C.c_int, C.POINTER(BuildOptions), C.POINTER(VP)], C.c_int),
+    "ljx_scene_bvh_info": ([VP, C.POINTER(BvhInfo)], C.c_int),
     "ljx_scene_destroy": ([VP], C.c_int),
     "ljx_scene_bounds": ([VP, C.POINTER(C.c_double)], C.c_int),
     "ljx_render": ([VP, C.POINTER(RenderOptions), C.POINTER(ExecOptions),

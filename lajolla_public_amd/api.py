@@ -18,12 +18,18 @@ from .scenes import SceneDesc, render_options
 class Scene:
     """Device scene: geometry, SAH BVH, light tables, owned by one handle."""
 
-    def __init__(self, desc: SceneDesc, device: int = 0):
+    def __init__(self, desc: SceneDesc, device: int = 0, bvh_builder=None, max_leaf_size=0):
+        """bvh_builder: None (the library default, the device builder),
+        capi.BVH_BUILDER_GPU or capi.BVH_BUILDER_HOST."""
         self.lib = capi.load()
         self.desc = desc
         d, keep = desc.to_c()
         h = C.c_void_p()
-        capi.check(self.lib.ljx_scene_create(C.byref(d), device, C.byref(h)), self.lib)
+        if bvh_builder is None and not max_leaf_size:
+            capi.check(self.lib.ljx_scene_create(C.byref(d), device, C.byref(h)), self.lib)
+        else:
+            bo = capi.BuildOptions(capi.BVH_BUILDER_GPU if bvh_builder is None else bvh_builder, max_leaf_size)
+            capi.check(self.lib.ljx_scene_create_ex(C.byref(d), device, C.byref(bo), C.byref(h)), self.lib)
         self.h = h
         self.device = device
         self.width = desc.camera["width"]
@@ -39,6 +45,11 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+    def bvh_info(self):
+        info = capi.BvhInfo()
+        capi.check(self.lib.ljx_scene_bvh_info(self.h, C.byref(info)), self.lib)
+        return {k: getattr(info, k) for k, _ in capi.BvhInfo._fields_}
 
     def bounds(self):
         out = (C.c_double * 4)()

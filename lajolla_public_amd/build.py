@@ -15,7 +15,7 @@ OUT = os.path.join(HERE, "liblajolla_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["ljx_lib.hip", "ljx_guide_net.hip"]
+HIP_SOURCES = ["ljx_lib.hip", "ljx_guide_net.hip", "ljx_bvh_gpu.hip"]
 CXX_SOURCES = ["ljx_bvh.cpp", "ljx_parse.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",

@@ -1,9 +1,13 @@
 // Binned-SAH BVH builder (host).  Top-down, 32 bins per axis over primitive
 // centroids; subtrees above a size threshold are built on separate threads
 // into private node arrays and spliced, so the output is independent of the
-// thread count.  Deep ranges fall back to object-median splits so the tree
-// depth stays bounded (kMaxTreeDepth).
+// thread count.  Split decisions come from ljx_bvh_sah.h, which the device
+// builder (ljx_bvh_gpu.hip) shares, and partitions are stable, so both build
+// the same tree.  Ranges without a binned split (all centroids equal, or
+// deeper than kBvhMedianDepth) are halved in order, which bounds the depth
+// (kMaxTreeDepth).
 #include "ljx_bvh.h"
+#include "ljx_bvh_sah.h"
 #include "ljx_trace.h"
 
 #include <algorithm>
@@ -17,9 +21,7 @@ namespace ljx {
 
 namespace {
 
-constexpr int kBins = 32;
 constexpr float kInf = std::numeric_limits<float>::infinity();
-constexpr int kMedianDepth = 80;
 
 struct Box {
     float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
@@ -29,11 +31,22 @@ struct Box {
             hi[k] = std::max(hi[k], h[k]);
         }
     }
-    void grow(const Box &b) { grow(b.lo, b.hi); }
-    float half_area() const {
-        float d0 = hi[0] - lo[0], d1 = hi[1] - lo[1], d2 = hi[2] - lo[2];
-        if (!(d0 >= 0)) return 0;
-        return d0 * d1 + d1 * d2 + d2 * d0;
+    float half_area() const { return box_half_area(lo, hi); }
+};
+
+struct HostBins {
+    int cnt[3][kBvhBins];
+    float lo[3][kBvhBins][3], hi[3][kBvhBins][3];
+    HostBins() {
+        for (int k = 0; k < 3; k++)
+            for (int b = 0; b < kBvhBins; b++) {
+                cnt[k][b] = 0;
+                for (int d = 0; d < 3; d++) lo[k][b][d] = kInf, hi[k][b][d] = -kInf;
+            }
+    }
+    __host__ __device__ int count(int k, int b) const { return cnt[k][b]; }
+    __host__ __device__ void box(int k, int b, float l[3], float h[3]) const {
+        for (int d = 0; d < 3; d++) l[d] = lo[k][b][d], h[d] = hi[k][b][d];
     }
 };
 
@@ -54,8 +67,6 @@ struct Builder {
         }
     }
 
-    static int leaf_ref(int first, int cnt) { return ~((first << 3) | (cnt - 1)); }
-
     // Builds [begin, end) into out (local node indices); returns the child
     // reference and writes the range's box.
     int build(int begin, int end, int depth, std::vector<BvhNode> &out, Box &box_out) {
@@ -71,79 +82,37 @@ struct Builder {
         int cur = max_depth.load();
         while (depth > cur && !max_depth.compare_exchange_weak(cur, depth)) {
         }
-        if (n <= 1) return leaf_ref(begin, n);
+        if (n <= 1) return bvh_leaf_ref(begin, n);
 
-        int axis = -1, split_bin = -1;
-        float best = kInf;
         float ext[3];
         for (int k = 0; k < 3; k++) ext[k] = cb.hi[k] - cb.lo[k];
-        if (depth < kMedianDepth) {
-            int cnt[3][kBins] = {};
-            Box bins[3][kBins];
+        SahSplit split;
+        if (depth < kBvhMedianDepth) {
+            HostBins bins;
             for (int i = begin; i < end; i++) {
                 const float *c = &cen[3 * ids[i]];
                 const PrimBox &p = boxes[ids[i]];
                 for (int k = 0; k < 3; k++) {
                     if (!(ext[k] > 0)) continue;
-                    int b = std::min(kBins - 1, int(kBins * (c[k] - cb.lo[k]) / ext[k]));
-                    cnt[k][b]++;
-                    bins[k][b].grow(p.lo, p.hi);
+                    int b = sah_bin(c[k], cb.lo[k], ext[k]);
+                    bins.cnt[k][b]++;
+                    box_grow(bins.lo[k][b], bins.hi[k][b], p.lo, p.hi);
                 }
             }
-            for (int k = 0; k < 3; k++) {
-                if (!(ext[k] > 0)) continue;
-                float rarea[kBins];
-                int rcnt[kBins];
-                Box acc;
-                int c = 0;
-                for (int b = kBins - 1; b > 0; b--) {
-                    acc.grow(bins[k][b]);
-                    c += cnt[k][b];
-                    rarea[b] = acc.half_area();
-                    rcnt[b] = c;
-                }
-                Box lacc;
-                c = 0;
-                for (int b = 0; b < kBins - 1; b++) {
-                    lacc.grow(bins[k][b]);
-                    c += cnt[k][b];
-                    if (c == 0 || rcnt[b + 1] == 0) continue;
-                    float cost = c * lacc.half_area() + rcnt[b + 1] * rarea[b + 1];
-                    if (cost < best) {
-                        best = cost;
-                        axis = k;
-                        split_bin = b;
-                    }
-                }
-            }
+            for (int k = 0; k < 3; k++)
+                if (ext[k] > 0) sah_take(split, sah_axis_best(bins, k));
         }
-        float pa = bb.half_area();
-        // SAH in units of one primitive test; a node visit costs one test.
-        float split_cost = pa > 0 ? 1.0f + best / pa : kInf;
-        float leaf_cost = float(n);
-        if (n <= max_leaf && (axis < 0 || leaf_cost <= split_cost)) return leaf_ref(begin, n);
+        if (sah_make_leaf(n, max_leaf, split, bb.half_area())) return bvh_leaf_ref(begin, n);
 
         int mid;
-        if (axis >= 0) {
-            int *b0 = ids.data() + begin;
-            int *b1 = ids.data() + end;
-            int *m = std::partition(b0, b1, [&](int id) {
-                float c = cen[3 * id + axis];
-                int b = std::min(kBins - 1, int(kBins * (c - cb.lo[axis]) / ext[axis]));
-                return b <= split_bin;
-            });
-            mid = int(m - ids.data());
+        if (split.axis >= 0) {
+            const int axis = split.axis, sb = split.bin;
+            const float clo = cb.lo[axis], cext = ext[axis];
+            std::stable_partition(ids.begin() + begin, ids.begin() + end,
+                                  [&](int id) { return sah_bin(cen[3 * id + axis], clo, cext) <= sb; });
+            mid = begin + split.nleft;
         } else {
-            mid = begin;
-        }
-        if (mid == begin || mid == end) {
-            // object median on the widest centroid axis
-            int k = 0;
-            if (ext[1] > ext[k]) k = 1;
-            if (ext[2] > ext[k]) k = 2;
             mid = begin + n / 2;
-            std::nth_element(ids.begin() + begin, ids.begin() + mid, ids.begin() + end,
-                             [&](int a, int b) { return cen[3 * a + k] < cen[3 * b + k]; });
         }
 
         int me = int(out.size());
@@ -188,68 +157,26 @@ struct Builder {
 
 namespace {
 
-struct Child {
-    int ref;
-    float lo[3], hi[3];
-    float half_area() const {
-        float d0 = hi[0] - lo[0], d1 = hi[1] - lo[1], d2 = hi[2] - lo[2];
-        return d0 * d1 + d1 * d2 + d2 * d0;
-    }
-};
-
-void binary_children(const BvhNode &nd, Child &a, Child &b) {
-    a.ref = nd.ch.x;
-    a.lo[0] = nd.lxy.x, a.hi[0] = nd.lxy.y, a.lo[1] = nd.lxy.z, a.hi[1] = nd.lxy.w, a.lo[2] = nd.zz.x, a.hi[2] = nd.zz.y;
-    b.ref = nd.ch.y;
-    b.lo[0] = nd.rxy.x, b.hi[0] = nd.rxy.y, b.lo[1] = nd.rxy.z, b.hi[1] = nd.rxy.w, b.lo[2] = nd.zz.z, b.hi[2] = nd.zz.w;
-}
-
 struct Collapser {
     const std::vector<BvhNode> &bin;
     std::vector<Bvh4Node> out;
     int depth = 0, stack_need = 0;
 
-    // Emits the 4-wide node for binary node b; returns its index.  level and
-    // stacked: depth of this node and stack entries held on arrival.
+    // Emits the 4-wide node for binary node b in depth-first preorder;
+    // returns its index.  level and stacked: depth of this node and stack
+    // entries held on arrival.
     int emit(int b, int level, int stacked) {
-        std::vector<Child> kids(2);
-        binary_children(bin[b], kids[0], kids[1]);
-        while (kids.size() < 4) {
-            int best = -1;
-            float area = -1;
-            for (size_t i = 0; i < kids.size(); i++)
-                if (kids[i].ref >= 0 && kids[i].half_area() > area) area = kids[i].half_area(), best = int(i);
-            if (best < 0) break;
-            Child a, c;
-            binary_children(bin[kids[best].ref], a, c);
-            kids[best] = a;
-            kids.insert(kids.begin() + best + 1, c);
-        }
+        Kid kids[4];
+        int nk = open4(bin.data(), b, kids);
         int me = int(out.size());
         out.emplace_back();
         depth = std::max(depth, level);
         // a traversal entering one child may have pushed all the others
-        int held = stacked + int(kids.size()) - 1;
+        int held = stacked + nk - 1;
         stack_need = std::max(stack_need, held);
-        float lo[3][4], hi[3][4];
-        int ch[4];
-        for (int c = 0; c < 4; c++) {
-            for (int k = 0; k < 3; k++) lo[k][c] = kInf, hi[k][c] = -kInf;
-            ch[c] = kNoChild;
-        }
-        for (size_t c = 0; c < kids.size(); c++) {
-            for (int k = 0; k < 3; k++) lo[k][c] = kids[c].lo[k], hi[k][c] = kids[c].hi[k];
-            ch[c] = kids[c].ref >= 0 ? emit(kids[c].ref, level + 1, held) : kids[c].ref;
-        }
-        Bvh4Node &nd = out[me];
-        nd.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
-        nd.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
-        nd.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
-        nd.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
-        nd.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
-        nd.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
-        nd.ch = make_int4(ch[0], ch[1], ch[2], ch[3]);
-        nd.meta = make_int4(int(kids.size()), 0, 0, 0);
+        for (int c = 0; c < nk; c++)
+            if (kids[c].ref >= 0) kids[c].ref = emit(kids[c].ref, level + 1, held);
+        out[me] = make_bvh4(kids, nk);
         return me;
     }
 };

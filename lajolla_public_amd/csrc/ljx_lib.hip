@@ -2,6 +2,7 @@
 // upload, BVH build, the wavefront render loop and the parity entry points.
 #include "../../include/lajolla_amd.h"
 #include "ljx_bvh.h"
+#include "ljx_bvh_gpu.h"
 #include "ljx_guide_net.h"
 #include "ljx_guide_render.h"
 #include "ljx_host.h"
@@ -210,6 +211,13 @@ struct Scene {
     float box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};  // axis-aligned bounds of every primitive
     int bvh_depth = 0;
     int spill_entries = 1;  // per-lane global stack entries beyond the LDS ones
+    int stack_need = 0;     // most traversal stack entries a ray can hold
+    int binary_depth = 0;
+    int bvh_builder = LJX_BVH_BUILDER_GPU;
+    int max_leaf = 8;
+    int num_slots = 0;
+    const int *slot_prim = nullptr;  // device: leaf slot -> primitive
+    double bvh_ms = 0;
     double build_ms = 0;
     std::unique_ptr<RenderCtx> rc;
     int rc_alloc_W = 0, rc_alloc_H = 0;
@@ -358,6 +366,115 @@ void validate(const ljx_scene_desc *d) {
         fail(LJX_ERR_INVALID_ARGUMENT, "envmap light id out of range");
 }
 
+// Primitive boxes on the host, as the device computes them (signed zeros made
+// positive): triangles from the global vertex positions, then spheres.
+std::vector<PrimBox> host_prim_boxes(const std::vector<float> &vpos, const std::vector<int> &tri_vidx, int ntri,
+                                     const std::vector<double4> &spheres) {
+    std::vector<PrimBox> boxes(size_t(ntri) + spheres.size());
+    for (int t = 0; t < ntri; t++) {
+        const float *p0 = &vpos[3 * tri_vidx[3 * t]], *p1 = &vpos[3 * tri_vidx[3 * t + 1]],
+                    *p2 = &vpos[3 * tri_vidx[3 * t + 2]];
+        for (int k = 0; k < 3; k++) {
+            boxes[t].lo[k] = std::min(p0[k], std::min(p1[k], p2[k])) + 0.0f;
+            boxes[t].hi[k] = std::max(p0[k], std::max(p1[k], p2[k])) + 0.0f;
+        }
+    }
+    for (size_t i = 0; i < spheres.size(); i++) {
+        const double4 &c = spheres[i];
+        const double cc[3] = {c.x, c.y, c.z};
+        for (int k = 0; k < 3; k++) {
+            boxes[ntri + i].lo[k] = float(cc[k] - c.w) + 0.0f;
+            boxes[ntri + i].hi[k] = float(cc[k] + c.w) + 0.0f;
+        }
+    }
+    return boxes;
+}
+
+// Builds the 4-wide BVH over the scene's primitives (on the device, or on the
+// host when the scene asks for it; both give the same tree), then the leaf-slot
+// records and the shading bucket of every slot.  The geometry arrays are
+// already on the device.
+void build_bvh(Scene &sc, const ljx_scene_desc *d, const std::vector<MatRec> &mats, int ntri, int nsph,
+               const std::vector<float> &vpos, const std::vector<int> &tri_vidx,
+               const std::vector<double4> &spheres) {
+    DeviceScene &S = sc.ds;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int nprim = ntri + nsph;
+    sc.num_slots = nprim;
+    if (nprim == 0) {
+        S.nodes = sc.upload(std::vector<Bvh4Node>());
+        S.num_nodes = 0;
+        S.prims = sc.upload(std::vector<float4>());
+        S.num_slots = 0;
+        S.slot_bucket = sc.upload(std::vector<int8_t>());
+        sc.slot_prim = sc.upload(std::vector<int>());
+        sc.bvh_depth = 0;
+        sc.spill_entries = 1;
+        return;
+    }
+    auto keep = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        HIPX(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+        sc.bufs.push_back(DevBuf{p, bytes});
+        return p;
+    };
+    int stack_need = 0;
+    if (sc.bvh_builder == LJX_BVH_BUILDER_GPU) {
+        float4 *box = nullptr;
+        HIPX(hipMalloc(&box, sizeof(float4) * 2 * size_t(nprim)));
+        struct Free {
+            void *p;
+            ~Free() { hipFree(p); }
+        } fbox{box};
+        prim_boxes_device(S.vpos, S.tri_vidx, ntri, S.spheres, nsph, box, box + nprim, sc.stream);
+        DeviceBvh dv;
+        build_bvh_device(box, box + nprim, nprim, sc.max_leaf, sc.stream, keep, dv);
+        S.nodes = dv.nodes;
+        S.num_nodes = dv.num_nodes;
+        sc.slot_prim = dv.slot_prim;
+        sc.bvh_depth = dv.depth;
+        sc.binary_depth = dv.max_binary_depth;
+        stack_need = dv.stack_need;
+    } else {
+        BvhBuildResult bvh = build_bvh_sah(host_prim_boxes(vpos, tri_vidx, ntri, spheres), sc.max_leaf, 0);
+        Bvh4Result bvh4 = collapse_bvh4(bvh);
+        S.nodes = sc.upload(bvh4.nodes);
+        S.num_nodes = int(bvh4.nodes.size());
+        sc.slot_prim = sc.upload(bvh.slot_prim);
+        sc.bvh_depth = bvh4.depth;
+        sc.binary_depth = bvh.max_depth;
+        stack_need = bvh4.stack_need;
+    }
+    sc.stack_need = stack_need;
+    sc.spill_entries = std::max(1, stack_need - kStackLds);
+    if (sc.spill_entries > kMaxStackSpill) fail(LJX_ERR_UNSUPPORTED, "BVH needs a deeper traversal stack than this build has");
+    // the bucket a closest hit on each slot joins when hits are sorted by
+    // material: its material type, or the runtime-dispatch bucket for
+    // textured materials, or the no-material bucket
+    std::vector<int8_t> shape_bucket(d->num_shapes);
+    for (int i = 0; i < d->num_shapes; i++) {
+        const int mid = d->shapes[i].material_id;
+        shape_bucket[i] = int8_t(mid < 0 ? kBucketNoMat : (mats[mid].textured ? kBucketAny : mats[mid].type));
+    }
+    std::vector<DevBuf> tmp;
+    const int8_t *sb_dev = sc.upload(shape_bucket, &tmp);
+    struct FreeAll {
+        std::vector<DevBuf> &v;
+        ~FreeAll() {
+            for (auto &b : v) hipFree(b.p);
+        }
+    } ftmp{tmp};
+    float4 *prims = static_cast<float4 *>(keep(sizeof(float4) * 3 * size_t(nprim)));
+    int8_t *slot_bucket = static_cast<int8_t *>(keep(size_t(nprim)));
+    slot_prims_device(sc.slot_prim, nprim, S.vpos, S.tri_vidx, ntri, S.spheres, prims, sc.stream);
+    slot_buckets_device(sc.slot_prim, nprim, S.prim_shape, sb_dev, slot_bucket, sc.stream);
+    HIPX(hipStreamSynchronize(sc.stream));
+    S.prims = prims;
+    S.num_slots = nprim;
+    S.slot_bucket = slot_bucket;
+    sc.bvh_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 void build_scene(Scene &sc, const ljx_scene_desc *d) {
     validate(d);
     auto t0 = std::chrono::steady_clock::now();
@@ -375,10 +492,19 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
     std::vector<float> vpos, vnrm, vuv;
     std::vector<int> tri_vidx, prim_shape, prim_local;
     std::vector<ShapeRec> shapes(d->num_shapes);
-    std::vector<PrimBox> boxes;
     int ntri = 0;
+    size_t nvert = 0;
     for (int i = 0; i < d->num_shapes; i++)
-        if (d->shapes[i].type == LJX_SHAPE_TRIANGLE_MESH) ntri += int(d->shapes[i].num_triangles);
+        if (d->shapes[i].type == LJX_SHAPE_TRIANGLE_MESH) {
+            ntri += int(d->shapes[i].num_triangles);
+            nvert += size_t(d->shapes[i].num_vertices);
+        }
+    vpos.reserve(3 * nvert);
+    vnrm.reserve(3 * nvert);
+    vuv.reserve(2 * nvert);
+    tri_vidx.reserve(3 * size_t(ntri));
+    prim_shape.reserve(size_t(ntri) + size_t(d->num_shapes));
+    prim_local.reserve(size_t(ntri) + size_t(d->num_shapes));
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     std::vector<double> tri_cdf;
     std::vector<char> light_shape(d->num_shapes, 0);
@@ -411,33 +537,33 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
                 vuv.push_back(s.uvs ? s.uvs[2 * v] : 0.f);
                 vuv.push_back(s.uvs ? s.uvs[2 * v + 1] : 0.f);
             }
-            // triangle areas in FP64 (src/shapes/triangle_mesh.inl:60-75)
-            std::vector<double> areas(s.num_triangles);
-            double total = 0;
             for (uint64_t t = 0; t < s.num_triangles; t++) {
-                int a = s.indices[3 * t], b = s.indices[3 * t + 1], c = s.indices[3 * t + 2];
-                tri_vidx.push_back(a + voff);
-                tri_vidx.push_back(b + voff);
-                tri_vidx.push_back(c + voff);
+                tri_vidx.push_back(s.indices[3 * t] + voff);
+                tri_vidx.push_back(s.indices[3 * t + 1] + voff);
+                tri_vidx.push_back(s.indices[3 * t + 2] + voff);
                 prim_shape.push_back(i);
                 prim_local.push_back(int(t));
-                PrimBox pb;
+            }
+            // triangle areas in FP64 (src/shapes/triangle_mesh.inl:60-75): the
+            // light's pdf and its triangle table, so emitting shapes only
+            const bool emits = s.area_light_id >= 0 || light_shape[i];
+            std::vector<double> areas(emits ? s.num_triangles : 0);
+            double total = 0;
+            for (uint64_t t = 0; emits && t < s.num_triangles; t++) {
+                int a = s.indices[3 * t], b = s.indices[3 * t + 1], c = s.indices[3 * t + 2];
                 double e1[3], e2[3];
                 for (int k = 0; k < 3; k++) {
                     float pa = s.positions[3 * a + k], pbk = s.positions[3 * b + k], pc = s.positions[3 * c + k];
-                    pb.lo[k] = std::min(pa, std::min(pbk, pc));
-                    pb.hi[k] = std::max(pa, std::max(pbk, pc));
                     e1[k] = double(pbk) - double(pa);
                     e2[k] = double(pc) - double(pa);
                 }
-                boxes.push_back(pb);
                 double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
                        cz = e1[0] * e2[1] - e1[1] * e2[0];
                 areas[t] = std::sqrt(cx * cx + cy * cy + cz * cz) / 2;
                 total += areas[t];
             }
             r.total_area = total;
-            if ((s.area_light_id >= 0 || light_shape[i]) && s.num_triangles > 0) {
+            if (emits && s.num_triangles > 0) {
                 std::vector<double> pmf, cdf;
                 make_dist(areas, pmf, cdf);
                 r.cdf_offset = int(tri_cdf.size());
@@ -459,18 +585,15 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
         spheres.push_back(make_double4(s.center[0], s.center[1], s.center[2], s.radius));
         prim_shape.push_back(i);
         prim_local.push_back(0);
-        PrimBox pb;
         for (int k = 0; k < 3; k++) {
-            pb.lo[k] = float(s.center[k] - s.radius);
-            pb.hi[k] = float(s.center[k] + s.radius);
-            lo[k] = std::min(lo[k], pb.lo[k]);
-            hi[k] = std::max(hi[k], pb.hi[k]);
+            lo[k] = std::min(lo[k], float(s.center[k] - s.radius));
+            hi[k] = std::max(hi[k], float(s.center[k] + s.radius));
         }
-        boxes.push_back(pb);
         nsph++;
     }
     // ---- scene bounds (src/scene.cpp:30-34)
-    if (boxes.empty()) {
+    const int nprim = ntri + nsph;
+    if (nprim == 0) {
         for (int k = 0; k < 3; k++) lo[k] = hi[k] = 0;
     }
     double dx = double(hi[0]) - lo[0], dy = double(hi[1]) - lo[1], dz = double(hi[2]) - lo[2];
@@ -482,33 +605,6 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
     }
     S.bound_radius = sc.bound_radius;
     S.eps = std::min(sc.bound_radius * 1e-5, 0.01);
-
-    // ---- BVH
-    BvhBuildResult bvh = build_bvh_sah(boxes, 8, 0);
-    Bvh4Result bvh4 = collapse_bvh4(bvh);
-    sc.bvh_depth = bvh4.depth;
-    sc.spill_entries = std::max(1, bvh4.stack_need - kStackLds);
-    if (sc.spill_entries > kMaxStackSpill) fail(LJX_ERR_UNSUPPORTED, "BVH needs a deeper traversal stack than this build has");
-    std::vector<float4> prims(bvh.slot_prim.size() * 3);
-    for (size_t slot = 0; slot < bvh.slot_prim.size(); slot++) {
-        int gid = bvh.slot_prim[slot];
-        if (gid < ntri) {
-            const int *vi = &tri_vidx[3 * gid];
-            for (int k = 0; k < 3; k++) {
-                const float *p = &vpos[3 * vi[k]];
-                float w = k == 0 ? __builtin_bit_cast(float, gid) : (k == 1 ? __builtin_bit_cast(float, kPrimTri) : 0.f);
-                prims[3 * slot + k] = make_float4(p[0], p[1], p[2], w);
-            }
-        } else {
-            const ljx_shape_desc &s = d->shapes[prim_shape[gid]];
-            int si = gid - ntri;
-            prims[3 * slot] = make_float4(float(s.center[0]), float(s.center[1]), float(s.center[2]),
-                                          __builtin_bit_cast(float, gid));
-            prims[3 * slot + 1] = make_float4(__builtin_bit_cast(float, si), 0.f, 0.f,
-                                              __builtin_bit_cast(float, kPrimSphere));
-            prims[3 * slot + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-    }
 
     // ---- materials
     std::vector<MatRec> mats(d->num_materials);
@@ -665,28 +761,16 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
     // ---- camera (src/camera.cpp:7-21)
     S.camera = make_camera(d->camera);
 
-    // ---- upload
-    S.nodes = sc.upload(bvh4.nodes);
-    S.num_nodes = int(bvh4.nodes.size());
-    S.prims = sc.upload(prims);
-    S.spheres = sc.upload(spheres);
-    S.num_slots = int(bvh.slot_prim.size());
+    // ---- geometry to the device
     S.vpos = sc.upload(vpos);
     S.vnrm = sc.upload(vnrm);
     S.vuv = sc.upload(vuv);
     S.tri_vidx = sc.upload(tri_vidx);
     S.prim_shape = sc.upload(prim_shape);
-    {
-        // the bucket a closest hit on each slot joins when hits are sorted by
-        // material: its material type, or the runtime-dispatch bucket for
-        // textured materials, or the no-material bucket
-        std::vector<int8_t> sb(bvh.slot_prim.size());
-        for (size_t slot = 0; slot < sb.size(); slot++) {
-            const int mid = d->shapes[prim_shape[bvh.slot_prim[slot]]].material_id;
-            sb[slot] = int8_t(mid < 0 ? kBucketNoMat : (mats[mid].textured ? kBucketAny : mats[mid].type));
-        }
-        S.slot_bucket = sc.upload(sb);
-    }
+    S.spheres = sc.upload(spheres);
+
+    // ---- BVH (the HIGH-quality build of rtcCommitScene, src/scene.cpp:20-27)
+    build_bvh(sc, d, mats, ntri, nsph, vpos, tri_vidx, spheres);
     S.prim_local = sc.upload(prim_local);
     S.shapes = sc.upload(shapes);
     S.materials = sc.upload(mats);
@@ -1645,6 +1729,10 @@ int ljx_device_count(int *count) {
 }
 
 int ljx_scene_create(const ljx_scene_desc *desc, int device, void **scene_out) {
+    return ljx_scene_create_ex(desc, device, nullptr, scene_out);
+}
+
+int ljx_scene_create_ex(const ljx_scene_desc *desc, int device, const ljx_build_options *build, void **scene_out) {
     return guarded([&]() {
         if (!scene_out) fail(LJX_ERR_INVALID_ARGUMENT, "scene_out is NULL");
         *scene_out = nullptr;
@@ -1658,8 +1746,78 @@ int ljx_scene_create(const ljx_scene_desc *desc, int device, void **scene_out) {
         HIPX(hipGetDeviceProperties(&prop, device));
         sc->num_cu = prop.multiProcessorCount;
         HIPX(hipStreamCreateWithFlags(&sc->stream, hipStreamNonBlocking));
+        if (build) {
+            if (build->bvh_builder != LJX_BVH_BUILDER_GPU && build->bvh_builder != LJX_BVH_BUILDER_HOST)
+                fail(LJX_ERR_INVALID_ARGUMENT, "unknown BVH builder");
+            if (build->max_leaf_size < 0 || build->max_leaf_size > 8)
+                fail(LJX_ERR_INVALID_ARGUMENT, "max_leaf_size must be in [0, 8]");
+            sc->bvh_builder = build->bvh_builder;
+            if (build->max_leaf_size > 0) sc->max_leaf = build->max_leaf_size;
+        }
         build_scene(*sc, desc);
         *scene_out = sc.release();
+    });
+}
+
+int ljx_scene_bvh_info(void *scene, ljx_bvh_info *info) {
+    return guarded([&]() {
+        Scene *sc = as_scene(scene);
+        if (!info) fail(LJX_ERR_INVALID_ARGUMENT, "info is NULL");
+        HIPX(hipSetDevice(sc->device));
+        std::memset(info, 0, sizeof(*info));
+        info->builder = sc->bvh_builder;
+        info->depth = sc->bvh_depth;
+        info->stack_need = sc->stack_need;
+        info->binary_depth = sc->binary_depth;
+        info->num_nodes = uint64_t(sc->ds.num_nodes);
+        info->num_slots = uint64_t(sc->num_slots);
+        info->bvh_ms = sc->bvh_ms;
+        info->build_ms = sc->build_ms;
+        std::vector<Bvh4Node> nodes(sc->ds.num_nodes);
+        std::vector<int> slots(sc->num_slots);
+        if (!nodes.empty())
+            HIPX(hipMemcpy(nodes.data(), sc->ds.nodes, nodes.size() * sizeof(Bvh4Node), hipMemcpyDeviceToHost));
+        if (!slots.empty())
+            HIPX(hipMemcpy(slots.data(), sc->slot_prim, slots.size() * sizeof(int), hipMemcpyDeviceToHost));
+        auto fnv = [](const void *p, size_t n) {
+            uint64_t h = 1469598103934665603ull;
+            const unsigned char *c = static_cast<const unsigned char *>(p);
+            for (size_t i = 0; i < n; i++) h = (h ^ c[i]) * 1099511628211ull;
+            return h;
+        };
+        info->node_hash = fnv(nodes.data(), nodes.size() * sizeof(Bvh4Node));
+        info->slot_hash = fnv(slots.data(), slots.size() * sizeof(int));
+        // SAH cost with unit node and primitive costs: every node (by the
+        // area of its box as its parent stores it, relative to the root's)
+        // plus every leaf's primitive count times its relative area
+        if (!nodes.empty()) {
+            auto area = [](double lx, double hx, double ly, double hy, double lz, double hz) {
+                double a = hx - lx, b = hy - ly, c = hz - lz;
+                return a * b + b * c + c * a;
+            };
+            auto child = [&](const Bvh4Node &nd, int c, double &a) {
+                const float *lox = &nd.lox.x, *hix = &nd.hix.x, *loy = &nd.loy.x, *hiy = &nd.hiy.x,
+                            *loz = &nd.loz.x, *hiz = &nd.hiz.x;
+                a = area(lox[c], hix[c], loy[c], hiy[c], loz[c], hiz[c]);
+                return (&nd.ch.x)[c];
+            };
+            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            const Bvh4Node &r = nodes[0];
+            for (int c = 0; c < r.meta.x; c++) {
+                lo[0] = std::min<double>(lo[0], (&r.lox.x)[c]), hi[0] = std::max<double>(hi[0], (&r.hix.x)[c]);
+                lo[1] = std::min<double>(lo[1], (&r.loy.x)[c]), hi[1] = std::max<double>(hi[1], (&r.hiy.x)[c]);
+                lo[2] = std::min<double>(lo[2], (&r.loz.x)[c]), hi[2] = std::max<double>(hi[2], (&r.hiz.x)[c]);
+            }
+            const double root = area(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]);
+            double cost = 1;
+            for (const Bvh4Node &nd : nodes)
+                for (int c = 0; c < nd.meta.x; c++) {
+                    double a;
+                    int ref = child(nd, c, a);
+                    cost += (ref >= 0 ? 1.0 : double((~ref & 7) + 1)) * (root > 0 ? a / root : 1.0);
+                }
+            info->sah_cost = cost;
+        }
     });
 }
 

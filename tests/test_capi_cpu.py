@@ -34,7 +34,7 @@ STRUCTS = ["ljx_shape_desc", "ljx_material_desc", "ljx_light_desc", "ljx_medium_
            "ljx_texture_desc", "ljx_camera_desc",
            "ljx_render_options", "ljx_scene_desc", "ljx_ray", "ljx_hit", "ljx_bsdf_query", "ljx_bsdf_result",
            "ljx_render_stats", "ljx_exec_options", "ljx_guide_config", "ljx_guide_options", "ljx_batch_job",
-           "ljx_batch_stats"]
+           "ljx_batch_stats", "ljx_build_options", "ljx_bvh_info"]
 MIRROR = {"ljx_shape_desc": capi.ShapeDesc, "ljx_material_desc": capi.MaterialDesc, "ljx_light_desc": capi.LightDesc,
           "ljx_medium_desc": capi.MediumDesc, "ljx_volume_desc": capi.VolumeDesc,
           "ljx_texture_desc": capi.TextureDesc, "ljx_camera_desc": capi.CameraDesc,
@@ -42,7 +42,8 @@ MIRROR = {"ljx_shape_desc": capi.ShapeDesc, "ljx_material_desc": capi.MaterialDe
           "ljx_hit": capi.Hit, "ljx_bsdf_query": capi.BSDFQuery, "ljx_bsdf_result": capi.BSDFResult,
           "ljx_render_stats": capi.RenderStats, "ljx_exec_options": capi.ExecOptions,
           "ljx_guide_config": capi.GuideConfig, "ljx_guide_options": capi.GuideOptions,
-          "ljx_batch_job": capi.BatchJob, "ljx_batch_stats": capi.BatchStats}
+          "ljx_batch_job": capi.BatchJob, "ljx_batch_stats": capi.BatchStats,
+          "ljx_build_options": capi.BuildOptions, "ljx_bvh_info": capi.BvhInfo}
 
 
 def test_struct_sizes_match_c(tmp_path):
