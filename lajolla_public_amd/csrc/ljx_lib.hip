@@ -789,7 +789,9 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
 }
 
 // ------------------------------------------------------------ trace kernels
-template <bool ANY>
+// SPH: the scene has spheres; without, the triangle-only traversal the
+// render's closest-hit and any-hit kernels run in such scenes.
+template <bool ANY, bool SPH>
 __global__ __launch_bounds__(256) void k_trace_batch(DeviceScene S, const ljx_ray *rays, size_t n, ljx_hit *hits,
                                                    uint8_t *occ, int *spill, unsigned long long *stat, int stats) {
     __shared__ int stack[kStackLds * 256];
@@ -802,11 +804,11 @@ __global__ __launch_bounds__(256) void k_trace_batch(DeviceScene S, const ljx_ra
         bool ok = false;
         if (S.num_nodes > 0) {
             if (stats)
-                ok = trace_ray<ANY, true, true>(TraceScene{S.nodes, S.prims, S.spheres}, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1],
+                ok = trace_ray<ANY, true, SPH>(TraceScene{S.nodes, S.prims, S.spheres}, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1],
                                           r.dir[2], r.tnear, r.tfar, stack + threadIdx.x, blockDim.x,
                                           spill + gtid, stride, h, nn, np);
             else
-                ok = trace_ray<ANY, false, true>(TraceScene{S.nodes, S.prims, S.spheres}, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1],
+                ok = trace_ray<ANY, false, SPH>(TraceScene{S.nodes, S.prims, S.spheres}, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1],
                                            r.dir[2], r.tnear, r.tfar, stack + threadIdx.x, blockDim.x,
                                            spill + gtid, stride, h, nn, np);
         }
@@ -828,7 +830,7 @@ __global__ __launch_bounds__(256) void k_trace_batch(DeviceScene S, const ljx_ra
                 o.v = h.v;
                 o.shape_id = S.prim_shape[gid];
                 o.prim_id = S.prim_local[gid];
-                if (__float_as_int(p1.w) == kPrimTri) {
+                if (!SPH || __float_as_int(p1.w) == kPrimTri) {
                     float4 p2 = S.prims[3 * h.slot + 2];
                     float e1x = p1.x - p0.x, e1y = p1.y - p0.y, e1z = p1.z - p0.z;
                     float e2x = p2.x - p0.x, e2y = p2.y - p0.y, e2z = p2.z - p0.z;
@@ -1671,7 +1673,8 @@ void trace_device(Scene &sc, const ljx_ray *rays, size_t n, void *out, const ljx
     HIPX(hipEventCreate(&a));
     HIPX(hipEventCreate(&b));
     HIPX(hipEventRecord(a, sc.stream));
-    hipLaunchKernelGGL(k_trace_batch<ANY>, dim3(grid), dim3(256), 0, sc.stream, sc.ds, rays, n,
+    auto kern = sc.has_spheres ? k_trace_batch<ANY, true> : k_trace_batch<ANY, false>;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, sc.stream, sc.ds, rays, n,
                        ANY ? nullptr : static_cast<ljx_hit *>(out), ANY ? static_cast<uint8_t *>(out) : nullptr,
                        spill.p, stat, st_on);
     HIPX(hipGetLastError());
@@ -1911,8 +1914,9 @@ int ljx_trace_closest_device_noalloc(void *scene, const ljx_ray *rays, size_t n,
         // the same persistent grid as ljx_trace_closest_device, capped by what
         // the scratch holds; rays do not depend on the grid
         const int grid = int(std::min<size_t>(size_t(trace_grid_for(*sc)), kScratchInts / per_block));
-        hipLaunchKernelGGL(k_trace_batch<false>, dim3(grid), dim3(256), 0, sc->stream, sc->ds, rays, n, hits,
-                           nullptr, spill, nullptr, 0);
+        auto kern = sc->has_spheres ? k_trace_batch<false, true> : k_trace_batch<false, false>;
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, sc->stream, sc->ds, rays, n, hits, nullptr, spill, nullptr,
+                           0);
         HIPX(hipGetLastError());
         HIPX(hipStreamSynchronize(sc->stream));
     });

@@ -14,6 +14,9 @@
 //             seed = options.seed.  Draw order inside a path is the
 //             reference's; the HIP wavefront tracer uses the same plan, so
 //             the two agree sample by sample up to FP32/FP64 rounding.
+//   plan 2 -- plan 0 with the tile streams offset by the seed (stream =
+//             tile + options.seed * number of tiles): independent
+//             realisations of the reference's own renderer.
 #include "orc_core.h"
 #include "../include/lajolla_amd.h"
 
@@ -1624,7 +1627,7 @@ int orc_render(void *sp, const ljx_render_options *opts, int plan, int threads, 
         int tile = tile_begin + int(k);
         int tx = tile % ntx, ty = tile / ntx;
         Counters cnt;
-        Pcg rng = pcg_init(uint64_t(ty) * ntx + tx);
+        Pcg rng = pcg_init(uint64_t(ty) * ntx + tx + (plan == 2 ? s->options.seed * uint64_t(ntiles) : 0));
         int x0 = tx * ts, x1 = std::min(x0 + ts, w), y0 = ty * ts, y1 = std::min(y0 + ts, h);
         uint64_t paths = 0;
         for (int y = y0; y < y1; y++)

@@ -120,3 +120,23 @@ def test_terrain_spheres_hit_parity(n_rays):
     # ~0.12% of these rays land within 1e-6 (relative) of an edge of the
     # 0.04-unit terrain triangles seen from ~20 units away
     assert keep.mean() > 0.995
+    # any hit on the same rays as segments [1e-4, tfar]: tfar on either side
+    # of the exact closest hit (0.5 to 1.5 times its distance), or random for
+    # rays that miss; the scene has no spheres, so this is the triangle-only
+    # traversal the render's shadow rays run
+    r = hits_to_numpy(ref)
+    rng = np.random.default_rng(13)
+    hit = r["shape"] >= 0
+    tfar = np.where(hit, r["t"] * rng.uniform(0.5, 1.5, n_rays), rng.uniform(0.1, 40.0, n_rays)).astype(np.float32)
+    seg = make_rays(o, d, tnear=1e-4, tfar=tfar)
+    occ_g = sc.trace_any(seg)
+    occ_r = orc.trace_any(seg)
+    # excluded: segments ending within 1e-5 (relative) of the hit, edge rays,
+    # and hits closer than the segment start
+    ambiguous = hit & ((np.abs(r["t"] - tfar) <= 1e-5 * tfar) | (mb < 1e-6) | (r["t"] < 2e-4))
+    agree = (occ_g == occ_r) | ambiguous
+    print(f"terrain any hit: segments={n_rays} occluded={int(occ_r.sum())} excluded={int(ambiguous.sum())} "
+          f"mismatches={int((~agree).sum())}", flush=True)
+    assert agree.all(), np.nonzero(~agree)[0][:10]
+    # and the oracle's any hit is the closest hit's verdict
+    assert np.array_equal(occ_r[~ambiguous], (hit & (r["t"] <= tfar))[~ambiguous])
