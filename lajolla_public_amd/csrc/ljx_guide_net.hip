@@ -91,7 +91,11 @@ __global__ __launch_bounds__(256) void k_forward_raw(const float4 *feats, int n,
 
 // Forward and backward of 32-sample groups: stores every hidden activation
 // and every pre-activation gradient in the group layout above, and adds the
-// batch's weighted loss and weight sum to red[0], red[1].
+// batch's weighted loss and weight sum as per-wave partials (k_red_sum adds
+// them into red[0], red[1]).
+// Most blocks of k_fwdbwd (each leaves 8 loss partials for k_red_sum).
+constexpr int kMaxFwdGrid = 1024;
+
 __global__ __launch_bounds__(512) void k_fwdbwd(const float4 *feats, const float4 *dirs, int n, const bf16x8 *ffrag,
                                                const float *fbias, const bf16x8 *bfrag, uint16_t *acts, int groups,
                                                float *red) {
@@ -205,10 +209,25 @@ __global__ __launch_bounds__(512) void k_fwdbwd(const float4 *feats, const float
         loss += __shfl_xor(loss, off);
         wsum += __shfl_xor(wsum, off);
     }
+    // one partial per wave; k_red_sum adds them in a fixed order
     if (lane == 0) {
-        atomicAdd(&red[0], loss);
-        atomicAdd(&red[1], wsum);
+        red[4 + 2 * (blockIdx.x * 8 + w)] = loss;
+        red[5 + 2 * (blockIdx.x * 8 + w)] = wsum;
     }
+}
+
+// Sums the per-wave loss and weight partials of k_fwdbwd in wave order into
+// red[0], red[1] (a fixed order, so the normalisation of every gradient is
+// the same on every run).
+__global__ void k_red_sum(float *red, int nparts) {
+    if (threadIdx.x != 0) return;
+    float l = 0.f, ws = 0.f;
+    for (int p = 0; p < nparts; p++) {
+        l += red[4 + 2 * p];
+        ws += red[5 + 2 * p];
+    }
+    red[0] = l;
+    red[1] = ws;
 }
 
 // Weight gradient of layer L over a range of groups: dW^T = sum dZ^T H_in,
@@ -418,7 +437,7 @@ Net *net_create(int device, const ljx_guide_config &cfg) {
         dalloc(n->adam_m, kNumParams);
         dalloc(n->adam_v, kNumParams);
         dalloc(n->grad, kNumParams);
-        dalloc(n->red, 4);
+        dalloc(n->red, 4 + 2 * 8 * kMaxFwdGrid);
         dalloc(n->ffrag, size_t(kFwdFrags) * 64);
         dalloc(n->bfrag, size_t(kBwdFrags) * 64);
         dalloc(n->fbias, kBiasFwd);
@@ -474,9 +493,10 @@ void net_step(Net &n, const float4 *feats, const float4 *dirs, int count, bool u
     const int groups = n.batch_cap / 32;
     const int ngroups = (count + 31) / 32;
     hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, s, n.red, 4);
-    int fgrid = std::max(1, std::min((ngroups + 7) / 8, n.num_cu));
+    int fgrid = std::max(1, std::min({(ngroups + 7) / 8, n.num_cu, kMaxFwdGrid}));
     hipLaunchKernelGGL(k_fwdbwd, dim3(fgrid), dim3(512), 0, s, feats, dirs, count, n.ffrag, n.fbias, n.bfrag, n.acts,
                        groups, n.red);
+    hipLaunchKernelGGL(k_red_sum, dim3(1), dim3(64), 0, s, n.red, fgrid * 8);
     HIPX(hipGetLastError());
     int gpb = (ngroups + n.partial_blocks - 1) / n.partial_blocks;
     int nblk = (ngroups + gpb - 1) / gpb;

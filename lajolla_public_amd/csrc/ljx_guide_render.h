@@ -157,42 +157,96 @@ __global__ void k_guide_clear(float4 *a, float4 *b, float4 *c, size_t n) {
     }
 }
 
-__global__ void k_guide_collect(PathBufs P, float4 *out_feat, float4 *out_dir, int *count, int cap) {
+// Walks path i's recorded vertices from its last back to its first and
+// calls take(r, dir, weight) for each guided vertex with a positive finite
+// weight w = lum(radiance that arrived after the vertex / its throughput).
+template <class F>
+__device__ __forceinline__ void guide_path_samples(const PathBufs &P, int i, F &&take) {
+    const int last = P.depth[i];
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    for (int b = P.rec_depth - 1; b >= 0; b--) {
+        if (b > last) continue;  // vertices past the one the path ended at hold nothing of this pass
+        const size_t r = size_t(b) * P.rec_stride + i;
+        const float4 d = P.rec_dir[r];
+        if (d.w > 0.5f) {
+            const float4 be = P.rec_beta[r];
+            const float wr = be.x > 0.f ? sx / be.x : 0.f;
+            const float wg = be.y > 0.f ? sy / be.y : 0.f;
+            const float wb = be.z > 0.f ? sz / be.z : 0.f;
+            const float wgt = 0.212671f * wr + 0.715160f * wg + 0.072169f * wb;
+            if (wgt > 0.f && isfinite(wgt)) take(r, d, wgt);
+        }
+        const float4 c = P.rec_c[r];
+        sx += c.x, sy += c.y, sz += c.z;
+    }
+}
+
+// Training samples of a recording pass, in a fixed order (paths in slot
+// order, each path's vertices from last to first): the samples of each
+// block of 256 paths are counted, the counts scanned, then each block writes
+// its samples at its offset, so training sees the same batches on every run.
+__global__ __launch_bounds__(256) void k_guide_count(PathBufs P, int *blk_count) {
     const int n = P.pass_ns * P.pass_nq;
-    const int R = P.rec_depth;
-    int stride = gridDim.x * blockDim.x;
-    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
-        int i = base + threadIdx.x;
-        bool live = i < n;
-        // vertices past the one the path ended at hold nothing of this pass
-        const int last = live ? P.depth[i] : -1;
-        float sx = 0.f, sy = 0.f, sz = 0.f;
-        for (int b = R - 1; b >= 0; b--) {
-            size_t r = size_t(b) * P.rec_stride + (live ? i : 0);
-            bool take = false;
-            float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
-            float wgt = 0.f;
-            if (live && b <= last) {
-                d = P.rec_dir[r];
-                if (d.w > 0.5f) {
-                    float4 be = P.rec_beta[r];
-                    float wr = be.x > 0.f ? sx / be.x : 0.f;
-                    float wg = be.y > 0.f ? sy / be.y : 0.f;
-                    float wb = be.z > 0.f ? sz / be.z : 0.f;
-                    wgt = 0.212671f * wr + 0.715160f * wg + 0.072169f * wb;
-                    take = wgt > 0.f && isfinite(wgt);
-                }
-                float4 c = P.rec_c[r];
-                sx += c.x, sy += c.y, sz += c.z;
-            }
-            int k = wave_append(take, count);
-            if (take && k < cap) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int c = 0;
+    if (i < n) guide_path_samples(P, i, [&](size_t, float4, float) { c++; });
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    __shared__ int part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) blk_count[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Exclusive scan of the block counts in place; count[0] gets the total.
+__global__ __launch_bounds__(1024) void k_guide_scan(int *blk_count, int nblk, int *count) {
+    __shared__ int part[1024];
+    int carry = 0;
+    for (int base = 0; base < nblk; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = i < nblk ? blk_count[i] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int a = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += a;
+            __syncthreads();
+        }
+        if (i < nblk) blk_count[i] = carry + part[threadIdx.x] - v;
+        const int tot = part[1023];
+        __syncthreads();
+        carry += tot;
+    }
+    if (threadIdx.x == 0) count[0] = carry;
+}
+
+__global__ __launch_bounds__(256) void k_guide_collect(PathBufs P, const int *blk_off, float4 *out_feat,
+                                                       float4 *out_dir, int cap) {
+    const int n = P.pass_ns * P.pass_nq;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int c = 0;
+    if (i < n) guide_path_samples(P, i, [&](size_t, float4, float) { c++; });
+    // exclusive scan of the per-thread counts over the block, in thread order
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = c;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    __shared__ int wsum[4];
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int k = blk_off[blockIdx.x] + x - c;
+    for (int q = 0; q < w; q++) k += wsum[q];
+    if (i < n)
+        guide_path_samples(P, i, [&](size_t r, float4 d, float wgt) {
+            if (k < cap) {
                 out_feat[2 * size_t(k)] = P.rec_feat[2 * r];
                 out_feat[2 * size_t(k) + 1] = P.rec_feat[2 * r + 1];
                 out_dir[k] = make_float4(d.x, d.y, d.z, wgt);
             }
-        }
-    }
+            k++;
+        });
 }
 
 }  // namespace ljx

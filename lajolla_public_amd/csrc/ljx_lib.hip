@@ -175,6 +175,7 @@ struct GuideCtx {
            *rec_c = nullptr;
     float4 *s_feat = nullptr, *s_dir = nullptr;  // training samples of the last recording pass
     int *s_count = nullptr;
+    int *s_blk = nullptr;  // per block of 256 paths: sample count, then offset
     std::vector<DevBuf> bufs;
 };
 
@@ -1097,6 +1098,7 @@ void ensure_guide(Scene &sc, RenderCtx &rc, int rec_depth) {
     g.s_feat = sc.alloc<float4>(2 * size_t(g.sample_cap), g.bufs);
     g.s_dir = sc.alloc<float4>(size_t(g.sample_cap), g.bufs);
     g.s_count = sc.alloc<int>(4, g.bufs);
+    g.s_blk = sc.alloc<int>(C / 256 + 1, g.bufs);
 }
 
 void guide_infer(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s, const guide::Net &net) {
@@ -1303,8 +1305,10 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
             if (recording) {
                 // training samples of this pass, then Adam steps on them
                 tm.begin(5, ev);
-                launch(k_reset_counters, 1, 64, s, gc->s_count, 0, 1);
-                launch(k_guide_collect, shade_grid, 256, s, P, gc->s_feat, gc->s_dir, gc->s_count, gc->sample_cap);
+                const int cblk = (ns * nq + 255) / 256;
+                launch(k_guide_count, cblk, 256, s, P, gc->s_blk);
+                launch(k_guide_scan, 1, 1024, s, gc->s_blk, cblk, gc->s_count);
+                launch(k_guide_collect, cblk, 256, s, P, gc->s_blk, gc->s_feat, gc->s_dir, gc->sample_cap);
                 int cnt = 0;
                 HIPX(hipMemcpyAsync(&cnt, gc->s_count, sizeof(int), hipMemcpyDeviceToHost, s));
                 HIPX(hipStreamSynchronize(s));

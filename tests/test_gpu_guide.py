@@ -151,13 +151,30 @@ def rel_mse(img, ref, trim=0.0):
     return float(e.mean())
 
 
+def test_guided_training_is_reproducible():
+    """Training samples are gathered in path order and the loss normaliser is
+    summed in a fixed order, so two guided renders from the same network seed
+    train to the same parameters and give the same image, bit for bit."""
+    desc = gap_scene(W=96, H=54, spp=16)
+    sc = api.Scene(desc)
+    opts = scenes.render_options(**dict(desc.options, samples_per_pixel=16, seed=77))
+    out = []
+    for rep in range(2):
+        g = api.Guide(batch_size=1 << 16, seed=5)
+        img, st = api.render_guided(sc, g, opts, gopts=api.guide_options(train_spp=8, steps_per_pass=3), stats=True)
+        out.append((img, g.get_params(), st["train_samples"]))
+        g.close()
+    print(f"training samples {out[0][2]} and {out[1][2]}", flush=True)
+    assert out[0][2] == out[1][2] and out[0][2] > 0
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0], out[1][0])
+
+
 def test_trained_guide_lowers_error_at_equal_spp():
     """relMSE against a 8192-spp plain reference, four seeds each, at 32 spp:
     the standard figure must be lower (the guide finds the light through the
-    gap that the rare high-weight plain paths carry).  Without the 0.1% largest
-    pixel errors the two are within about a percent on this small image, and
-    the network's training is not bit-reproducible (its samples are gathered
-    with atomics), so that figure only has to stay within 2% of plain."""
+    gap that the rare high-weight plain paths carry), and so must the figure
+    without the 0.1% largest pixel errors."""
     desc = gap_scene(W=128, H=72, spp=32)
     sc = api.Scene(desc)
     ref = sc.render(options=scenes.render_options(**dict(desc.options, samples_per_pixel=8192, seed=999)))
@@ -176,4 +193,4 @@ def test_trained_guide_lowers_error_at_equal_spp():
     print(f"relMSE at 32 spp: plain {np.mean(ep):.5f} {np.round(ep, 4)}, guided {np.mean(eg):.5f} {np.round(eg, 4)}; "
           f"trimmed plain {np.mean(tp):.5f}, guided {np.mean(tg):.5f}", flush=True)
     assert np.mean(eg) < np.mean(ep)
-    assert np.mean(tg) < 1.02 * np.mean(tp)
+    assert np.mean(tg) < np.mean(tp)
