@@ -383,6 +383,8 @@ enum { LJX_BVH_BUILDER_GPU = 0, LJX_BVH_BUILDER_HOST = 1 };
 typedef struct ljx_build_options {
     int32_t bvh_builder;           /* LJX_BVH_BUILDER_* (default GPU) */
     int32_t max_leaf_size;         /* primitives per leaf at most, 1..8 (0: 8) */
+    float node_cost;               /* SAH cost of a node visit in primitive tests (0: the library default) */
+    int32_t pad;
 } ljx_build_options;
 
 /* ljx_scene_create with build options (NULL: the defaults). */

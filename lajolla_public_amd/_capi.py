@@ -301,7 +301,7 @@ class BatchJob(C.Structure):
 
 
 class BuildOptions(C.Structure):
-    _fields_ = [("bvh_builder", C.c_int32), ("max_leaf_size", C.c_int32)]
+    _fields_ = [("bvh_builder", C.c_int32), ("max_leaf_size", C.c_int32), ("node_cost", C.c_float), ("pad", C.c_int32)]
 
 
 class BvhInfo(C.Structure):

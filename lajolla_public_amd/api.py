@@ -18,17 +18,19 @@ from .scenes import SceneDesc, render_options
 class Scene:
     """Device scene: geometry, SAH BVH, light tables, owned by one handle."""
 
-    def __init__(self, desc: SceneDesc, device: int = 0, bvh_builder=None, max_leaf_size=0):
+    def __init__(self, desc: SceneDesc, device: int = 0, bvh_builder=None, max_leaf_size=0, node_cost=0.0):
         """bvh_builder: None (the library default, the device builder),
-        capi.BVH_BUILDER_GPU or capi.BVH_BUILDER_HOST."""
+        capi.BVH_BUILDER_GPU or capi.BVH_BUILDER_HOST; max_leaf_size and
+        node_cost: 0 for the library defaults."""
         self.lib = capi.load()
         self.desc = desc
         d, keep = desc.to_c()
         h = C.c_void_p()
-        if bvh_builder is None and not max_leaf_size:
+        if bvh_builder is None and not max_leaf_size and not node_cost:
             capi.check(self.lib.ljx_scene_create(C.byref(d), device, C.byref(h)), self.lib)
         else:
-            bo = capi.BuildOptions(capi.BVH_BUILDER_GPU if bvh_builder is None else bvh_builder, max_leaf_size)
+            bo = capi.BuildOptions(capi.BVH_BUILDER_GPU if bvh_builder is None else bvh_builder, max_leaf_size,
+                                   node_cost, 0)
             capi.check(self.lib.ljx_scene_create_ex(C.byref(d), device, C.byref(bo), C.byref(h)), self.lib)
         self.h = h
         self.device = device

@@ -55,10 +55,12 @@ struct Builder {
     std::vector<float> cen;  // 3 per primitive
     std::vector<int> ids;
     int max_leaf;
+    float node_cost;
     std::atomic<int> threads_left;
     std::atomic<int> max_depth{0};
 
-    Builder(const std::vector<PrimBox> &b, int ml, int threads) : boxes(b), max_leaf(ml), threads_left(threads) {
+    Builder(const std::vector<PrimBox> &b, int ml, float nc, int threads)
+        : boxes(b), max_leaf(ml), node_cost(nc), threads_left(threads) {
         cen.resize(b.size() * 3);
         ids.resize(b.size());
         for (size_t i = 0; i < b.size(); i++) {
@@ -102,7 +104,7 @@ struct Builder {
             for (int k = 0; k < 3; k++)
                 if (ext[k] > 0) sah_take(split, sah_axis_best(bins, k));
         }
-        if (sah_make_leaf(n, max_leaf, split, bb.half_area())) return bvh_leaf_ref(begin, n);
+        if (sah_make_leaf(n, max_leaf, split, bb.half_area(), node_cost)) return bvh_leaf_ref(begin, n);
 
         int mid;
         if (split.axis >= 0) {
@@ -195,12 +197,12 @@ Bvh4Result collapse_bvh4(const BvhBuildResult &bin) {
     return r;
 }
 
-BvhBuildResult build_bvh_sah(const std::vector<PrimBox> &boxes, int max_leaf, int threads) {
+BvhBuildResult build_bvh_sah(const std::vector<PrimBox> &boxes, int max_leaf, float node_cost, int threads) {
     BvhBuildResult res;
     if (boxes.empty()) return res;
     if (max_leaf < 1 || max_leaf > 8) max_leaf = 8;
     if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
-    Builder b(boxes, max_leaf, threads);
+    Builder b(boxes, max_leaf, node_cost, threads);
     std::vector<BvhNode> nodes;
     nodes.reserve(boxes.size());
     Box root_box;

@@ -20,8 +20,9 @@ struct BvhBuildResult {
     int max_depth = 0;
 };
 
-// Builds over n primitive boxes.  max_leaf <= 8.  threads <= 0: all cores.
-BvhBuildResult build_bvh_sah(const std::vector<PrimBox> &boxes, int max_leaf, int threads);
+// Builds over n primitive boxes.  max_leaf <= 8; node_cost: the SAH cost of
+// a node visit in primitive tests.  threads <= 0: all cores.
+BvhBuildResult build_bvh_sah(const std::vector<PrimBox> &boxes, int max_leaf, float node_cost, int threads);
 
 struct Bvh4Result {
     std::vector<Bvh4Node> nodes;  // node 0 is the root

@@ -22,11 +22,12 @@ struct DeviceBvh {
     double ms = 0;              // wall time of the build
 };
 
-// Builds over n primitive boxes (lo, hi; w unused).  keep(bytes) allocates
+// Builds over n primitive boxes (lo, hi; w unused); max_leaf and node_cost
+// as for build_bvh_sah (ljx_bvh.h).  keep(bytes) allocates
 // device memory that outlives the build (the nodes and the slot table);
 // temporaries are freed before returning.  Work is queued on stream, with a
 // host wait per level.
-void build_bvh_device(const float4 *box_lo, const float4 *box_hi, int n, int max_leaf, hipStream_t stream,
+void build_bvh_device(const float4 *box_lo, const float4 *box_hi, int n, int max_leaf, float node_cost, hipStream_t stream,
                       const std::function<void *(size_t)> &keep, DeviceBvh &out);
 
 // Boxes of the scene's primitives: triangles 0..ntri-1 from the global vertex

@@ -181,6 +181,7 @@ struct BuildCtx {
     BvhNode *nodes;
     int *counters;
     int max_leaf;
+    float node_cost;
 };
 // counters: binary nodes, next wide ranges, narrow ranges, max depth, stack
 // (narrow-range overflow, then the traversal stack need), level size, error
@@ -350,7 +351,7 @@ __global__ __launch_bounds__(64) void k_wide_split(BuildCtx B, const Item *items
     if (lane != 0) return;
     atomicMax(&B.counters[CN_DEPTH], it.depth);
     LevelSplit ls{-1, -1, it.begin, 0.f, 0.f};
-    if (n <= 1 || sah_make_leaf(n, B.max_leaf, s, box_half_area(bl, bh))) {
+    if (n <= 1 || sah_make_leaf(n, B.max_leaf, s, box_half_area(bl, bh), B.node_cost)) {
         // (wide ranges exceed max_leaf, so this does not happen; kept for completeness)
         write_child(B.nodes, it.parent, bl, bh, bvh_leaf_ref(it.begin, n));
         ls.mid = -1;
@@ -556,7 +557,7 @@ __global__ __launch_bounds__(64) void k_narrow(BuildCtx B, const Item *items, in
             __syncthreads();
             s = wave_sah(bins, ext[0], ext[1], ext[2]);
         }
-        if (sah_make_leaf(n, B.max_leaf, s, box_half_area(bl, bh))) {
+        if (sah_make_leaf(n, B.max_leaf, s, box_half_area(bl, bh), B.node_cost)) {
             if (lane == 0) write_child(B.nodes, it.parent, bl, bh, bvh_leaf_ref(base + it.begin, n));
             continue;
         }
@@ -816,7 +817,7 @@ void slot_buckets_device(const int *slot_prim, int n, const int *prim_shape, con
     HIPX(hipGetLastError());
 }
 
-void build_bvh_device(const float4 *box_lo, const float4 *box_hi, int n, int max_leaf, hipStream_t st,
+void build_bvh_device(const float4 *box_lo, const float4 *box_hi, int n, int max_leaf, float node_cost, hipStream_t st,
                       const std::function<void *(size_t)> &keep, DeviceBvh &out) {
     auto t0 = std::chrono::steady_clock::now();
     out = DeviceBvh{};
@@ -853,6 +854,7 @@ void build_bvh_device(const float4 *box_lo, const float4 *box_hi, int n, int max
     B.cen = cen;
     B.ids = static_cast<int *>(keep(size_t(n) * sizeof(int)));
     B.max_leaf = max_leaf;
+    B.node_cost = node_cost;
     HIPX(hipMemsetAsync(B.nodes, 0, sizeof(BvhNode) * nb_cap, st));
     int init_ctr[CN_NUM] = {1, 0, 0, 0, 0, 0, 0, 0};  // node 0 is the root
     HIPX(hipMemcpyAsync(B.counters, init_ctr, sizeof(init_ctr), hipMemcpyHostToDevice, st));

@@ -102,10 +102,10 @@ __host__ __device__ inline void sah_take(SahSplit &best, const SahSplit &s) {
 }
 
 // Leaf or split: SAH in units of one primitive test, a node visit costing
-// one test; ranges above max_leaf always split.
-__host__ __device__ inline bool sah_make_leaf(int n, int max_leaf, const SahSplit &s, float half_area) {
+// node_cost tests; ranges above max_leaf always split.
+__host__ __device__ inline bool sah_make_leaf(int n, int max_leaf, const SahSplit &s, float half_area, float node_cost) {
 #pragma clang fp contract(off)
-    float split_cost = half_area > 0 ? 1.0f + s.cost / half_area : __builtin_inff();
+    float split_cost = half_area > 0 ? node_cost + s.cost / half_area : __builtin_inff();
     float leaf_cost = float(n);
     return n <= max_leaf && (s.axis < 0 || leaf_cost <= split_cost);
 }

@@ -216,6 +216,7 @@ struct Scene {
     int binary_depth = 0;
     int bvh_builder = LJX_BVH_BUILDER_GPU;
     int max_leaf = 8;
+    float node_cost = 1.0f;  // SAH cost of a node visit in primitive tests
     int num_slots = 0;
     const int *slot_prim = nullptr;  // device: leaf slot -> primitive
     double bvh_ms = 0;
@@ -429,7 +430,7 @@ void build_bvh(Scene &sc, const ljx_scene_desc *d, const std::vector<MatRec> &ma
         } fbox{box};
         prim_boxes_device(S.vpos, S.tri_vidx, ntri, S.spheres, nsph, box, box + nprim, sc.stream);
         DeviceBvh dv;
-        build_bvh_device(box, box + nprim, nprim, sc.max_leaf, sc.stream, keep, dv);
+        build_bvh_device(box, box + nprim, nprim, sc.max_leaf, sc.node_cost, sc.stream, keep, dv);
         S.nodes = dv.nodes;
         S.num_nodes = dv.num_nodes;
         sc.slot_prim = dv.slot_prim;
@@ -437,7 +438,7 @@ void build_bvh(Scene &sc, const ljx_scene_desc *d, const std::vector<MatRec> &ma
         sc.binary_depth = dv.max_binary_depth;
         stack_need = dv.stack_need;
     } else {
-        BvhBuildResult bvh = build_bvh_sah(host_prim_boxes(vpos, tri_vidx, ntri, spheres), sc.max_leaf, 0);
+        BvhBuildResult bvh = build_bvh_sah(host_prim_boxes(vpos, tri_vidx, ntri, spheres), sc.max_leaf, sc.node_cost, 0);
         Bvh4Result bvh4 = collapse_bvh4(bvh);
         S.nodes = sc.upload(bvh4.nodes);
         S.num_nodes = int(bvh4.nodes.size());
@@ -1760,6 +1761,8 @@ int ljx_scene_create_ex(const ljx_scene_desc *desc, int device, const ljx_build_
                 fail(LJX_ERR_INVALID_ARGUMENT, "max_leaf_size must be in [0, 8]");
             sc->bvh_builder = build->bvh_builder;
             if (build->max_leaf_size > 0) sc->max_leaf = build->max_leaf_size;
+            if (build->node_cost < 0 || !(build->node_cost < 64)) fail(LJX_ERR_INVALID_ARGUMENT, "node_cost out of range");
+            if (build->node_cost > 0) sc->node_cost = build->node_cost;
         }
         build_scene(*sc, desc);
         *scene_out = sc.release();
