@@ -234,6 +234,7 @@ struct DeviceScene {
     const int *tri_vidx;      // 3 global vertex ids per global triangle
     const int *prim_shape;    // shape id per global primitive id
     const int8_t *slot_bucket; // per leaf slot: shading bucket of its material when hits are sorted
+    const int8_t *slot_passthru; // per leaf slot: 1 for an index-matched medium boundary (no material)
     const int *prim_local;    // primitive id inside its shape
     const ShapeRec *shapes;
     const MatRec *materials;

@@ -409,6 +409,7 @@ void build_bvh(Scene &sc, const ljx_scene_desc *d, const std::vector<MatRec> &ma
         S.prims = sc.upload(std::vector<float4>());
         S.num_slots = 0;
         S.slot_bucket = sc.upload(std::vector<int8_t>());
+        S.slot_passthru = sc.upload(std::vector<int8_t>());
         sc.slot_prim = sc.upload(std::vector<int>());
         sc.bvh_depth = 0;
         sc.spill_entries = 1;
@@ -453,13 +454,15 @@ void build_bvh(Scene &sc, const ljx_scene_desc *d, const std::vector<MatRec> &ma
     // the bucket a closest hit on each slot joins when hits are sorted by
     // material: its material type, or the runtime-dispatch bucket for
     // textured materials, or the no-material bucket
-    std::vector<int8_t> shape_bucket(d->num_shapes);
+    std::vector<int8_t> shape_bucket(d->num_shapes), shape_passthru(d->num_shapes);
     for (int i = 0; i < d->num_shapes; i++) {
         const int mid = d->shapes[i].material_id;
         shape_bucket[i] = int8_t(mid < 0 ? kBucketNoMat : (mats[mid].textured ? kBucketAny : mats[mid].type));
+        shape_passthru[i] = int8_t(mid < 0 ? 1 : 0);
     }
     std::vector<DevBuf> tmp;
     const int8_t *sb_dev = sc.upload(shape_bucket, &tmp);
+    const int8_t *sp_dev = sc.upload(shape_passthru, &tmp);
     struct FreeAll {
         std::vector<DevBuf> &v;
         ~FreeAll() {
@@ -468,12 +471,15 @@ void build_bvh(Scene &sc, const ljx_scene_desc *d, const std::vector<MatRec> &ma
     } ftmp{tmp};
     float4 *prims = static_cast<float4 *>(keep(sizeof(float4) * 3 * size_t(nprim)));
     int8_t *slot_bucket = static_cast<int8_t *>(keep(size_t(nprim)));
+    int8_t *slot_passthru = static_cast<int8_t *>(keep(size_t(nprim)));
     slot_prims_device(sc.slot_prim, nprim, S.vpos, S.tri_vidx, ntri, S.spheres, prims, sc.stream);
     slot_buckets_device(sc.slot_prim, nprim, S.prim_shape, sb_dev, slot_bucket, sc.stream);
+    slot_buckets_device(sc.slot_prim, nprim, S.prim_shape, sp_dev, slot_passthru, sc.stream);
     HIPX(hipStreamSynchronize(sc.stream));
     S.prims = prims;
     S.num_slots = nprim;
     S.slot_bucket = slot_bucket;
+    S.slot_passthru = slot_passthru;
     sc.bvh_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 

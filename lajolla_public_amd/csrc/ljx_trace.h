@@ -173,18 +173,21 @@ struct TraceScene {
     const Bvh4Node *__restrict__ nodes;
     const float4 *__restrict__ prims;    // 3 per leaf slot
     const double4 *__restrict__ spheres; // centre xyz, radius
+    const int8_t *__restrict__ skip = nullptr;  // OPAQUE traversals: per slot, nonzero to pass through
 };
 
 // Traversal state of one ray, advanced one inner node or one leaf per step.
 // stack: this lane's LDS column base (entries at stack[k*stride]); spill:
 // this lane's global spill column.  SPH: the scene has sphere primitives
-// (otherwise every leaf slot is a triangle).
-template <bool ANY, bool STATS, bool SPH>
+// (otherwise every leaf slot is a triangle).  OPAQUE (any hit only): slots
+// that ts.skip marks are passed through (index-matched medium boundaries).
+template <bool ANY, bool STATS, bool SPH, bool OPAQUE = false>
 struct Tracer {
     RayPre r;
     float dx, dy, dz, tmin, tmax;
     int node, sp;
     bool found;
+    bool passed;  // OPAQUE: a marked slot was hit and passed through
     TraceHit hit;
 
     __device__ __forceinline__ void start(float ox, float oy, float oz, float dx_, float dy_, float dz_, float tmin_,
@@ -198,6 +201,7 @@ struct Tracer {
         node = 0;  // the root is an inner node
         sp = 0;
         found = false;
+        passed = false;
         hit.slot = -1;
     }
 
@@ -267,6 +271,10 @@ struct Tracer {
                     ok = sphere_roots(r.ox, r.oy, r.oz, dx, dy, dz, ts.spheres[__float_as_int(p1.x)], tmin, tmax, td);
                     t = float(td);
                 }
+                if (OPAQUE && ok && ts.skip[slot]) {
+                    ok = false;
+                    passed = true;
+                }
                 if (ok) {
                     found = true;
                     hit.t = t;
@@ -309,11 +317,11 @@ __device__ __forceinline__ float refine_t(const float4 *__restrict__ prims, int 
 }
 
 // Trace one ray to the end.  Returns true on a hit.
-template <bool ANY, bool STATS, bool SPH>
+template <bool ANY, bool STATS, bool SPH, bool OPAQUE = false>
 __device__ __forceinline__ bool trace_ray(const TraceScene &ts, float ox, float oy, float oz, float dx, float dy,
                                           float dz, float tmin, float tmax, int *stack, int stride, int *spill,
                                           int spill_stride, TraceHit &hit, uint32_t &n_nodes, uint32_t &n_prims) {
-    Tracer<ANY, STATS, SPH> tr;
+    Tracer<ANY, STATS, SPH, OPAQUE> tr;
     tr.start(ox, oy, oz, dx, dy, dz, tmin, tmax);
     while (!tr.step(ts, stack, stride, spill, spill_stride, n_nodes, n_prims)) {
     }

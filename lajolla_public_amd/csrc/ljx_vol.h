@@ -375,11 +375,30 @@ __global__ __launch_bounds__(256) void k_vol_shadow(DeviceScene S, PathBufs P) {
         D3 p0 = p;
         bool blocked = false;
         int shadow_bounces = 0;
+        // a surface with a material anywhere on the way to the light blocks
+        // the sample whatever the media: one any-hit query that passes
+        // through index-matched boundaries settles that before the walk (the
+        // NEE stream is not written back, so skipping the walk leaves every
+        // other draw unchanged); if it met no boundary either, the walk's
+        // first closest-hit query, over the same interval, finds nothing
+        bool crossings = false;
+        if (S.num_nodes > 0) {
+            const double dl = sqrt(dist2(p, pp));
+            Tracer<true, STATS, SPH, true> tr;
+            tr.start(float(p.x), float(p.y), float(p.z), float(dir_light.x), float(dir_light.y), float(dir_light.z),
+                     float(S.eps), float((1 - S.eps) * dl));
+            const TraceScene ts{S.nodes, S.prims, S.spheres, S.slot_passthru};
+            while (!tr.step(ts, stack + threadIdx.x, blockDim.x, spill, stride, nn, np)) {
+            }
+            segs++;
+            if (tr.found) continue;
+            crossings = tr.passed;
+        }
         for (int seg = 0; seg < kMaxShadowSegments; seg++) {
             double dl = sqrt(dist2(p0, pp));
             TraceHit h;
-            segs++;
-            bool hit = S.num_nodes > 0 &&
+            if (crossings) segs++;
+            bool hit = crossings &&
                        trace_ray<false, STATS, SPH>(TraceScene{S.nodes, S.prims, S.spheres}, float(p0.x), float(p0.y),
                                                     float(p0.z), float(dir_light.x), float(dir_light.y),
                                                     float(dir_light.z), float(S.eps), float((1 - S.eps) * dl),
