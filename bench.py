@@ -141,6 +141,13 @@ def main():
     traffic = measured_traffic(args.workload)
     hbm_gbs = traffic / (avg_launch_ms / 1000) / 1e9 if traffic and avg_launch_ms > 0 else None
     pipeline = measured_pipeline(args.workload)
+    # the vector-memory pipeline's rate for incoherent fetches over this tree:
+    # whole nodes at random indices, every lane its own node
+    # (ljx_scene_node_fetch_rate); the traversal's node and primitive fetch
+    # rate beside it, in bytes
+    random_fetch_gbs = sc.node_fetch_rate() * node_bytes / 1e9
+    fetch_gbs = (ext_rays * (node_bytes * nodes_per_ray + 48 * tris_per_ray)) / (ext_ms / 1000) / 1e9 \
+        if ext_ms > 0 else 0.0
 
     cpu = None
     if not args.no_cpu:
@@ -199,11 +206,23 @@ def main():
             "nodes_per_ray": round(nodes_per_ray, 2),
             "prims_per_ray": round(tris_per_ray, 2),
             "pipeline": pipeline,
+            "fetch_pipe": {
+                "traversal_fetch_gbs": round(fetch_gbs, 1),
+                "random_fetch_gbs": round(random_fetch_gbs, 1),
+                "ratio": round(fetch_gbs / random_fetch_gbs, 4) if random_fetch_gbs > 0 else None,
+                "unit": "GB/s of node (node_bytes each) and primitive (48 B each) fetches",
+                "note": "random_fetch_gbs: ljx_scene_node_fetch_rate on this scene's BVH, every lane of every wave "
+                        "fetching whole nodes at random indices (no coherence), measured live. The traversal's "
+                        "fetch rate is set against it: a ratio near or above 1 says the traversal moves node data "
+                        "as fast as the vector-memory pipeline serves incoherent fetches (above 1: the shared upper "
+                        "levels of the tree hit in cache)",
+            },
             "note": "achieved and frac are HBM bytes measured by counters (traffic, per launch) over the live "
                     "HIP-event launch time, against the 8 TB/s HBM peak. algorithmic_gbs counts every node and "
                     "primitive fetch of every ray (48 B + node_bytes x nodes + 36 B x primitives per ray); the "
                     "caches serve most of those, so it is not an HBM figure. The unit that bounds the kernel is "
-                    "the vector-memory pipeline (pipeline: texture address / data units busy fractions).",
+                    "the vector-memory pipeline (pipeline: texture address / data units busy fractions); fetch_pipe sets the "
+                    "traversal's fetch rate beside that pipeline's measured rate for random node fetches.",
         },
         "cpu_baseline": cpu,
         "guiding": guiding,

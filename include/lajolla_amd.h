@@ -410,6 +410,15 @@ typedef struct ljx_bvh_info {
 /* Fills info; downloads the nodes, so it is for tests and tools, not loops. */
 int ljx_scene_bvh_info(void *scene, ljx_bvh_info *info);
 
+/* Measurement helper with no counterpart in the reference: the rate
+ * (fetches per second, best of four timed launches) at which the device
+ * reads whole 4-wide nodes of the scene's BVH at random, `lanes` lanes of
+ * every wave (1..64) each fetching its own node with the next index
+ * depending on the data loaded, on the traversal's grid: the rate the
+ * vector-memory pipeline sustains for a traversal's fetches when the lanes
+ * of a wave share no nodes.  Allocates 4 bytes; blocks until done. */
+int ljx_scene_node_fetch_rate(void *scene, int lanes, double *fetches_per_s);
+
 /* Frees every device buffer of the scene.  Replaces Scene::~Scene
  * (src/scene.cpp:55-59). */
 int ljx_scene_destroy(void *scene);

@@ -348,6 +348,7 @@ SIGNATURES = {
     "ljx_scene_create": ([C.POINTER(SceneDescC), C.c_int, C.POINTER(VP)], C.c_int),
     "ljx_scene_create_ex": ([C.POINTER(SceneDescC), C.c_int, C.POINTER(BuildOptions), C.POINTER(VP)], C.c_int),
     "ljx_scene_bvh_info": ([VP, C.POINTER(BvhInfo)], C.c_int),
+    "ljx_scene_node_fetch_rate": ([VP, C.c_int, C.POINTER(C.c_double)], C.c_int),
     "ljx_scene_destroy": ([VP], C.c_int),
     "ljx_scene_bounds": ([VP, C.POINTER(C.c_double)], C.c_int),
     "ljx_render": ([VP, C.POINTER(RenderOptions), C.POINTER(ExecOptions),

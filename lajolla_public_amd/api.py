@@ -53,6 +53,13 @@ class Scene:
         capi.check(self.lib.ljx_scene_bvh_info(self.h, C.byref(info)), self.lib)
         return {k: getattr(info, k) for k, _ in capi.BvhInfo._fields_}
 
+    def node_fetch_rate(self, lanes=64):
+        """Random whole-node fetches per second over this scene's BVH (the
+        vector-memory pipeline's rate for lanes that share no nodes)."""
+        r = C.c_double(0)
+        capi.check(self.lib.ljx_scene_node_fetch_rate(self.h, int(lanes), C.byref(r)), self.lib)
+        return r.value
+
     def bounds(self):
         out = (C.c_double * 4)()
         capi.check(self.lib.ljx_scene_bounds(self.h, out), self.lib)

@@ -93,6 +93,19 @@ LJX_HD uint32_t pcg_next(uint64_t &state, uint64_t inc) {
     uint32_t rot = uint32_t(old >> 59u);
     return (xs >> rot) | (xs << ((-rot) & 31));
 }
+// The PCG stream of a camera path from its id (sample * W * H + pixel): a
+// bijection on 63-bit values, two multiply-xorshift rounds with the SplitMix64
+// finaliser's constants (Steele, Lea and Flood 2014), masked to 63 bits so
+// that distinct paths keep distinct increments.  Consecutive ids used as
+// increments give correlated PCG streams: neighbouring paths' draws are
+// related, which leaves an image's mean alone but not its error.
+LJX_HD uint64_t path_stream(uint64_t id) {
+    const uint64_t m = (uint64_t(1) << 63) - 1;
+    uint64_t z = id & m;
+    z = ((z ^ (z >> 31)) * 0xbf58476d1ce4e5b9ULL) & m;
+    z = ((z ^ (z >> 27)) * 0x94d049bb133111ebULL) & m;
+    return z ^ (z >> 31);
+}
 LJX_HD uint64_t pcg_init_state(uint64_t stream, uint64_t seed) {
     uint64_t inc = pcg_inc(stream);
     uint64_t st = 0;

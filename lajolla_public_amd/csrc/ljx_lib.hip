@@ -1837,6 +1837,71 @@ int ljx_scene_bvh_info(void *scene, ljx_bvh_info *info) {
     });
 }
 
+// Random node fetches over the scene's 4-wide BVH: each of `lanes` lanes
+// of every wave reads whole nodes (the traversal's seven 16-byte loads) at
+// hashed indices, the next index depending on the data just loaded, on the
+// traversal's grid.  This is the traversal's memory access pattern with no
+// coherence between the lanes of a wave and no arithmetic in the way (a
+// traversal of rays that share no nodes, at the vector-memory pipeline's
+// rate).
+__device__ __forceinline__ uint32_t fetch_hash(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__global__ __launch_bounds__(256) void k_node_fetch(const Bvh4Node *__restrict__ nodes, uint32_t n, int iters,
+                                                    int lanes, float *sink) {
+    uint32_t h = fetch_hash(blockIdx.x * 256u + threadIdx.x);
+    float acc = 0.f;
+    if (int(threadIdx.x & 63) < lanes) {
+        for (int it = 0; it < iters; it++) {
+            h = fetch_hash(h + uint32_t(it));
+            const Bvh4Node *nd = nodes + h % n;
+            const float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
+            const int4 ch = nd->ch;
+            acc += lx.x + hx.y + ly.z + hy.w + lz.x + hz.y + float(ch.z & 1);
+            h ^= __float_as_uint(acc) & 1u;
+        }
+    }
+    if (acc == 1234.5f) sink[0] = acc;  // keeps the loads; the sum never matters
+}
+
+int ljx_scene_node_fetch_rate(void *scene, int lanes, double *fetches_per_s) {
+    return guarded([&]() {
+        Scene *sc = as_scene(scene);
+        if (!fetches_per_s) fail(LJX_ERR_INVALID_ARGUMENT, "fetches_per_s is NULL");
+        if (lanes < 1 || lanes > 64) fail(LJX_ERR_INVALID_ARGUMENT, "lanes must be in 1..64");
+        if (sc->ds.num_nodes <= 0) fail(LJX_ERR_INVALID_ARGUMENT, "the scene has no BVH nodes");
+        HIPX(hipSetDevice(sc->device));
+        float *sink = nullptr;
+        HIPX(hipMalloc(&sink, sizeof(float)));
+        hipEvent_t e0, e1;
+        HIPX(hipEventCreate(&e0));
+        HIPX(hipEventCreate(&e1));
+        const int grid = trace_grid_for(*sc), iters = 64;
+        float best = 0;
+        for (int rep = 0; rep < 5; rep++) {
+            HIPX(hipEventRecord(e0, 0));
+            hipLaunchKernelGGL(k_node_fetch, dim3(grid), dim3(256), 0, 0, sc->ds.nodes, uint32_t(sc->ds.num_nodes),
+                               iters, lanes, sink);
+            HIPX(hipGetLastError());
+            HIPX(hipEventRecord(e1, 0));
+            HIPX(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPX(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0 && (best == 0 || ms < best)) best = ms;  // the first launch warms up
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        HIPX(hipFree(sink));
+        const double fetches = double(grid) * 4 * lanes * iters;
+        *fetches_per_s = best > 0 ? fetches / (best * 1e-3) : 0.0;
+    });
+}
+
 int ljx_scene_destroy(void *scene) {
     return guarded([&]() { delete static_cast<Scene *>(scene); });
 }

@@ -127,6 +127,19 @@ inline Pcg pcg_init(uint64_t stream, uint64_t seed = kDefaultSeed) {
     pcg_next(s);
     return s;
 }
+// Plan 1's stream of a camera path: a bijection on 63-bit values of the path
+// id (sample * W * H + pixel), two multiply-xorshift rounds with the
+// SplitMix64 finaliser's constants (Steele, Lea and Flood 2014), masked to 63
+// bits so that distinct paths keep distinct PCG increments.  Consecutive ids
+// as increments give correlated PCG streams (neighbouring paths' draws are
+// related), which changes the error of an image though not its mean.
+inline uint64_t path_stream(uint64_t id) {
+    const uint64_t m = (uint64_t(1) << 63) - 1;
+    uint64_t z = id & m;
+    z = ((z ^ (z >> 31)) * 0xbf58476d1ce4e5b9ULL) & m;
+    z = ((z ^ (z >> 27)) * 0x94d049bb133111ebULL) & m;
+    return z ^ (z >> 31);
+}
 inline double pcg_double(Pcg &r) {  // pcg.h:60-68
     union {
         uint64_t u;

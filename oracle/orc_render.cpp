@@ -10,8 +10,8 @@
 //             (stream = tile_y * ntx + tile_x, default seed), consumed
 //             sequentially over the tile's pixels and samples
 //             (src/render.cpp:71-101).
-//   plan 1 -- one PCG stream per camera path: stream = s * W * H + pixel,
-//             seed = options.seed.  Draw order inside a path is the
+//   plan 1 -- one PCG stream per camera path: stream = path_stream(s * W *
+//             H + pixel) (orc_core.h), seed = options.seed.  Draw order inside a path is the
 //             reference's; the HIP wavefront tracer uses the same plan, so
 //             the two agree sample by sample up to FP32/FP64 rounding.
 //   plan 2 -- plan 0 with the tile streams offset by the seed (stream =
@@ -1635,7 +1635,7 @@ int orc_render(void *sp, const ljx_render_options *opts, int plan, int threads, 
                 V3 rad;
                 for (int smp = 0; smp < spp; smp++) {
                     V3 L;
-                    uint64_t stream = uint64_t(smp) * uint64_t(w) * uint64_t(h) + uint64_t(y) * w + x;
+                    uint64_t stream = path_stream(uint64_t(smp) * uint64_t(w) * uint64_t(h) + uint64_t(y) * w + x);
                     if (plan == 1) {
                         Pcg prng = pcg_init(stream, s->options.seed);
                         L = vol ? vol_path_tracing(*s, x, y, prng, true, stream, cnt) : path_tracing(*s, x, y, prng, cnt);
@@ -1674,7 +1674,7 @@ int orc_trace_samples(void *sp, const ljx_render_options *opts, const uint32_t *
     parallel_for((n + 63) / 64, threads, [&](size_t blk) {
         Counters cnt;
         for (size_t i = blk * 64; i < std::min(n, blk * 64 + 64); i++) {
-            uint64_t stream = uint64_t(sample_idx[i]) * uint64_t(w) * uint64_t(h) + pixel[i];
+            uint64_t stream = path_stream(uint64_t(sample_idx[i]) * uint64_t(w) * uint64_t(h) + pixel[i]);
             Pcg prng = pcg_init(stream, s->options.seed);
             int x = int(pixel[i] % uint32_t(w)), y = int(pixel[i] / uint32_t(w));
             V3 L = vol ? vol_path_tracing(*s, x, y, prng, true, stream, cnt) : path_tracing(*s, x, y, prng, cnt);

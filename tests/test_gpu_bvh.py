@@ -95,3 +95,16 @@ def test_single_triangle_and_one_wave_ranges():
         P = (c + rng.normal(scale=0.1, size=(n, 3, 3))).reshape(-1, 3).astype(np.float32)
         s.add_mesh(P, np.arange(3 * n, dtype=np.int32).reshape(n, 3), m)
         check_same(s, f"{n} triangles")
+
+
+def test_node_fetch_rate():
+    """The random node-fetch rate: positive, higher with every lane of a
+    wave fetching than with eight, and argument errors cross the boundary."""
+    sc = api.Scene(scenes.terrain_spheres(seed=7, width=64, height=64, spp=1, grid=(200, 100), n_spheres=8,
+                                          sphere_subdiv=3, n_lights=64))
+    r64, r8 = sc.node_fetch_rate(64), sc.node_fetch_rate(8)
+    print(f"node fetches/s: 64 lanes {r64:.4g}, 8 lanes {r8:.4g}", flush=True)
+    assert np.isfinite(r64) and r64 > r8 > 0
+    for bad in (0, 65):
+        with pytest.raises(capi.LjxError):
+            sc.node_fetch_rate(bad)

@@ -1,7 +1,7 @@
 """The configs at their real geometry, sample-matched to the oracle.
 
-Each render uses per-path PCG streams on both sides (stream = sample * W * H
-+ pixel), so a pixel's device estimate equals the oracle's up to FP32/FP64
+Each render uses per-path PCG streams on both sides (the stream of path id
+sample * W * H + pixel), so a pixel's device estimate equals the oracle's up to FP32/FP64
 rounding except where rounding flips a discrete decision:
 
   * the large-mesh scene with every one of its 2,308,746 triangles, its

@@ -171,18 +171,22 @@ def test_guided_training_is_reproducible():
 
 
 def test_trained_guide_lowers_error_at_equal_spp():
-    """relMSE against a 8192-spp plain reference, four seeds each, at 32 spp:
-    the standard figure must be lower (the guide finds the light through the
-    gap that the rare high-weight plain paths carry), and so must the figure
-    without the 0.1% largest pixel errors."""
-    desc = gap_scene(W=128, H=72, spp=32)
+    """relMSE against a 4096-spp plain reference, six seeds each, at 32 spp
+    on a 480x270 view of the gap scene, after one 64-spp training pass of 16
+    steps (8.3M training samples): the standard figure must be lower on
+    average (the guide finds the light through the gap that the rare
+    high-weight plain paths carry), and the figure without the 0.1% largest
+    pixel errors must be lower on every seed.  Smaller views give the network
+    too few training samples to beat plain sampling: at 128x72 (590k samples)
+    the two are equal within noise."""
+    desc = gap_scene(W=480, H=270, spp=32)
     sc = api.Scene(desc)
-    ref = sc.render(options=scenes.render_options(**dict(desc.options, samples_per_pixel=8192, seed=999)))
+    ref = sc.render(options=scenes.render_options(**dict(desc.options, samples_per_pixel=4096, seed=999)))
     g = api.Guide(batch_size=1 << 18, seed=5)
     opts = scenes.render_options(**dict(desc.options, samples_per_pixel=64))
-    api.render_guided(sc, g, opts, gopts=api.guide_options(train_spp=64, steps_per_pass=6))
+    api.render_guided(sc, g, opts, gopts=api.guide_options(train_spp=64, steps_per_pass=16))
     ep, eg, tp, tg = [], [], [], []
-    for k in range(4):
+    for k in range(6):
         o = scenes.render_options(**dict(desc.options, seed=200 + k))
         ip = sc.render(options=o)
         ig = api.render_guided(sc, g, o, gopts=api.guide_options(train_spp=0))
@@ -191,6 +195,7 @@ def test_trained_guide_lowers_error_at_equal_spp():
         tp.append(rel_mse(ip, ref, 1e-3))
         tg.append(rel_mse(ig, ref, 1e-3))
     print(f"relMSE at 32 spp: plain {np.mean(ep):.5f} {np.round(ep, 4)}, guided {np.mean(eg):.5f} {np.round(eg, 4)}; "
-          f"trimmed plain {np.mean(tp):.5f}, guided {np.mean(tg):.5f}", flush=True)
+          f"trimmed plain {np.mean(tp):.5f} {np.round(tp, 4)}, guided {np.mean(tg):.5f} {np.round(tg, 4)}",
+          flush=True)
     assert np.mean(eg) < np.mean(ep)
-    assert np.mean(tg) < np.mean(tp)
+    assert (np.array(tg) < np.array(tp)).all()

@@ -2,7 +2,7 @@
 
 Two checks per scene:
   * sample-matched: the device and the oracle use the same per-path PCG
-    streams (stream = sample * W * H + pixel), so each pixel's estimate agrees
+    streams (the stream of path id sample * W * H + pixel), so each pixel's estimate agrees
     up to FP32/FP64 rounding except where rounding flips a discrete decision
     (Russian roulette, a lobe choice, a grazing hit); at least 97% of pixels
     must agree to 1e-3 relative (or 1e-4 absolute);
