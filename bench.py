@@ -241,8 +241,9 @@ GUIDE_TRAIN_FLOP_PER_SAMPLE = 3 * GUIDE_FLOP_PER_QUERY  # forward, input deltas,
 def guided_mode(args):
     """Guided renders of terrain_gap at 1080p, 64 spp, depth 8, each step from
     an untrained network (8 recording passes, 6 Adam steps on 2^20-sample
-    batches after each).  Roofline of k_guide_infer: 50,176 FLOP per query
-    (one query per extension-queue entry, the whole wave runs the MLP)."""
+    batches after each).  Roofline of k_guide_infer: 50,176 FLOP per query,
+    one query per path that hit a guided material (the guided types'
+    material buckets; the kernel counts them)."""
     from lajolla_public_amd import api, scenes
     desc = scenes.terrain_gap(seed=7, spp=args.spp or 64)
     sc = api.Scene(desc)
@@ -264,11 +265,11 @@ def guided_mode(args):
     runs = [step(k + 1) for k in range(args.steps)]
     ms = sum(r[0] for r in runs) / len(runs)
     st = [r[1] for r in runs]
-    ext = sum(s["extension_rays"] for s in st)
+    queries = sum(s["guide_queries"] for s in st)
     g_ms = sum(s["guide_ms"] for s in st)
     t_ms = sum(s["train_ms"] for s in st)
     steps = sum(s["train_steps"] for s in st)
-    achieved = ext * GUIDE_FLOP_PER_QUERY / (g_ms / 1000) / 1e12
+    achieved = queries * GUIDE_FLOP_PER_QUERY / (g_ms / 1000) / 1e12
     line = {
         "metric": "Msamples/s (1080p, depth 8, guided, training included)",
         "value": round(W * H * spp / (ms / 1000) / 1e6, 3),
@@ -285,7 +286,8 @@ def guided_mode(args):
             "kernel": "k_guide_infer (MLP forward + lobes + sampling)", "bound": "mfma",
             "achieved": round(achieved, 1), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
-            "flop_per_query": GUIDE_FLOP_PER_QUERY, "queries_per_step": ext // len(st),
+            "flop_per_query": GUIDE_FLOP_PER_QUERY, "queries_per_step": queries // len(st),
+            "extension_rays_per_step": sum(s["extension_rays"] for s in st) // len(st),
         },
         "training": {"adam_steps_per_step": steps // len(st),
                      "samples_per_step": sum(s["train_step_samples"] for s in st) // len(st),
@@ -351,7 +353,7 @@ def rel_mse(img, ref, trim=0.0):
 TRIM = 1e-4  # the 0.01% largest per-pixel errors (207 pixels at 1080p)
 
 
-def guided_vs_plain(ref_spp, plain_spp=64, train_spp=8, steps_per_pass=6, plain_seeds=3):
+def guided_vs_plain(ref_spp, plain_spp=64, train_spp=8, steps_per_pass=6, plain_seeds=8):
     """relMSE at equal wall time, learned guiding vs the plain tracer, on the
     guiding workload (terrain_gap: the large-mesh scene under a ceiling with a
     5% gap, 90% of the light power behind it), 1920x1080, depth 8.  Each
@@ -399,6 +401,7 @@ def guided_vs_plain(ref_spp, plain_spp=64, train_spp=8, steps_per_pass=6, plain_
     p_ms = float(np.mean([p["ms"] for p in plain]))
     p_err = float(np.mean([p["relmse"] for p in plain]))
     p_trim = float(np.mean([p["relmse_trimmed"] for p in plain]))
+    p_med = float(np.median([p["relmse"] for p in plain]))
     curve = []
     for spp in (8, 16, 24, 32, 48, 64):
         ms, img, st = guided(spp, 100 + spp)
@@ -425,17 +428,19 @@ def guided_vs_plain(ref_spp, plain_spp=64, train_spp=8, steps_per_pass=6, plain_
 
     eq, eq_trim = at_time("relmse"), at_time("relmse_trimmed")
     t_match, t_match_trim = time_to("relmse", p_err), time_to("relmse_trimmed", p_trim)
+    t_match_med = time_to("relmse", p_med)
     return {
         "workload": "terrain_gap: terrain_spheres (2,308,746 triangles) under a ceiling with a 5% gap; a 1x1 "
                     "emitter above the gap carries 90% of the light power; 1920x1080, depth 8",
         "reference_spp": ref_spp,
         "plain": {"spp": plain_spp, "ms": round(p_ms, 1), "relmse": round(p_err, 6),
-                  "relmse_trimmed": round(p_trim, 6), "seeds": plain},
+                  "relmse_median": round(p_med, 6), "relmse_trimmed": round(p_trim, 6), "seeds": plain},
         "guided_curve": curve,
         "guided_relmse_at_equal_time": round(eq, 6),
         "relmse_ratio_plain_over_guided_equal_time": round(p_err / eq, 3),
         "guided_time_to_plain_relmse_ms": round(t_match, 1) if t_match else None,
         "time_ratio_guided_over_plain": round(t_match / p_ms, 3) if t_match else None,
+        "time_ratio_guided_over_plain_median": round(t_match_med / p_ms, 3) if t_match_med else None,
         "trimmed": {"trim_fraction": TRIM, "guided_relmse_at_equal_time": round(eq_trim, 6),
                     "relmse_ratio_plain_over_guided_equal_time": round(p_trim / eq_trim, 3),
                     "time_ratio_guided_over_plain": round(t_match_trim / p_ms, 3) if t_match_trim else None},

@@ -337,6 +337,7 @@ typedef struct ljx_render_stats {
     uint64_t train_samples;        /* training samples recorded */
     uint64_t train_steps;          /* Adam steps taken */
     uint64_t train_step_samples;   /* samples summed over the Adam steps' batches */
+    uint64_t guide_queries;        /* network evaluations of guided vertices (guided renders) */
 } ljx_render_stats;
 
 /* Render-call switches that do not change results. */

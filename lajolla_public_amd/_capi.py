@@ -248,6 +248,7 @@ class RenderStats(C.Structure):
         ("train_samples", C.c_uint64),
         ("train_steps", C.c_uint64),
         ("train_step_samples", C.c_uint64),
+        ("guide_queries", C.c_uint64),
     ]
 
 

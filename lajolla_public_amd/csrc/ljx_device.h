@@ -257,11 +257,12 @@ struct DeviceScene {
     const double *tex_data;   // every texture level's pixels
     const double *light_pmf;  // num_lights
     const double *light_cdf;  // num_lights + 1
+    const int *light_guide;   // light_guide_m + 1: guide table of light_cdf (sample_cdf_guided)
     const double *tri_cdf;    // concatenated per-light-shape triangle CDFs
     int num_lights;
     int envmap_light_id;
     int num_triangles;
-    int pad0;
+    int light_guide_m;        // a power of two
     CameraRec camera;
     double bound_radius;
     double eps;  // min(1e-5 * r, 0.01) (src/scene.h:99-105)
@@ -331,6 +332,37 @@ LJX_HD int sample_cdf(const double *cdf, int n, double u) {
     }
     int k = lo - 1;
     return k < 0 ? 0 : (k > n - 1 ? n - 1 : k);
+}
+
+// The same index through a guide table (the cutpoint method of Chen and
+// Asau, 1974): guide[k] is the upper bound of k / m in the CDF, so for u in
+// [k / m, (k + 1) / m) the first cdf > u lies in [guide[k], guide[k + 1]] and
+// the search runs there only (one or two steps when m is near n, against
+// log2(n) for the whole table).  m is a power of two, so k = floor(u m) is
+// exact.
+LJX_HD int sample_cdf_guided(const double *cdf, int n, double u, const int *guide, int m) {
+    int k = int(u * m);
+    k = k < 0 ? 0 : (k > m - 1 ? m - 1 : k);
+    int lo = guide[k], hi = guide[k + 1];
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid + 1;
+        else hi = mid;
+    }
+    int r = lo - 1;
+    return r < 0 ? 0 : (r > n - 1 ? n - 1 : r);
+}
+
+// Builds the guide table of a CDF of n1 = n + 1 entries for m (a power of
+// two) cells: guide[k] = the first index whose cdf exceeds k / m (n1 if
+// none), k = 0..m.
+inline void cdf_guide(const double *cdf, int n1, int m, int *guide) {
+    int i = 0;
+    for (int k = 0; k <= m; k++) {
+        const double x = double(k) / m;
+        while (i < n1 && cdf[i] <= x) i++;
+        guide[k] = i;
+    }
 }
 
 // Cosine hemisphere (src/material.cpp:4-11).

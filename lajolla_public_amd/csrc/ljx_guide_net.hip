@@ -216,18 +216,31 @@ __global__ __launch_bounds__(512) void k_fwdbwd(const float4 *feats, const float
     }
 }
 
-// Sums the per-wave loss and weight partials of k_fwdbwd in wave order into
-// red[0], red[1] (a fixed order, so the normalisation of every gradient is
-// the same on every run).
-__global__ void k_red_sum(float *red, int nparts) {
-    if (threadIdx.x != 0) return;
+// Sums the per-wave loss and weight partials of k_fwdbwd into red[0],
+// red[1] in a fixed order, so the normalisation of every gradient is the
+// same on every run: thread t adds partials t, t + 256, ... in turn, each
+// wave adds its threads' sums by a butterfly, and thread 0 adds the four
+// waves' sums.
+__global__ __launch_bounds__(256) void k_red_sum(float *red, int nparts) {
     float l = 0.f, ws = 0.f;
-    for (int p = 0; p < nparts; p++) {
+    for (int p = threadIdx.x; p < nparts; p += 256) {
         l += red[4 + 2 * p];
         ws += red[5 + 2 * p];
     }
-    red[0] = l;
-    red[1] = ws;
+    for (int off = 32; off > 0; off >>= 1) {
+        l += __shfl_xor(l, off);
+        ws += __shfl_xor(ws, off);
+    }
+    __shared__ float sl[4], sw[4];
+    if ((threadIdx.x & 63) == 0) {
+        sl[threadIdx.x >> 6] = l;
+        sw[threadIdx.x >> 6] = ws;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        red[0] = (sl[0] + sl[1]) + (sl[2] + sl[3]);
+        red[1] = (sw[0] + sw[1]) + (sw[2] + sw[3]);
+    }
 }
 
 // Weight gradient of layer L over a range of groups: dW^T = sum dZ^T H_in,
@@ -496,7 +509,7 @@ void net_step(Net &n, const float4 *feats, const float4 *dirs, int count, bool u
     int fgrid = std::max(1, std::min({(ngroups + 7) / 8, n.num_cu, kMaxFwdGrid}));
     hipLaunchKernelGGL(k_fwdbwd, dim3(fgrid), dim3(512), 0, s, feats, dirs, count, n.ffrag, n.fbias, n.bfrag, n.acts,
                        groups, n.red);
-    hipLaunchKernelGGL(k_red_sum, dim3(1), dim3(64), 0, s, n.red, fgrid * 8);
+    hipLaunchKernelGGL(k_red_sum, dim3(1), dim3(256), 0, s, n.red, fgrid * 8);
     HIPX(hipGetLastError());
     int gpb = (ngroups + n.partial_blocks - 1) / n.partial_blocks;
     int nblk = (ngroups + gpb - 1) / gpb;

@@ -1,6 +1,7 @@
 // Guided sampling inside the wavefront loop.
 //
-//   k_guide_infer    after k_extend: for each path that hit a guided material,
+//   k_guide_infer    after k_extend: for each path that hit a guided material
+//                    (the guided types' material buckets),
 //                    network features of the vertex, the MLP forward pass on
 //                    the matrix cores (64 paths per wave), the vertex's lobe
 //                    set for the shading kernel, and the strategy choice: with
@@ -42,16 +43,40 @@ __global__ __launch_bounds__(256) void k_guide_infer(DeviceScene S, PathBufs P, 
     __shared__ float sfeat[4][64 * kFeat];
     __shared__ float su[4][64][4];
     __shared__ int spid[4][64];
-    load_lds(sfrag, ffrag, kFwdFrags * 64, sbias, fbias, kBiasFwd);
-    const uint32_t *q = cur ? P.extq1 : P.extq0;
-    const int n = P.ctr[C_EXT + cur];
+    // The paths to query: the buckets of the guided material types (every
+    // path there is guided) and the runtime-dispatch bucket (checked per
+    // path), one index space over their entries.  Paths in other buckets
+    // never read their guide direction, so the network runs on guided
+    // vertices only.
+    __shared__ int bstart[kNumBuckets + 1];
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int B = 0; B < kNumBuckets; B++) {
+            bstart[B] = acc;
+            const bool sel = B == kBucketAny || (B < kNumMatTypes && guidable(B) && ((P.guide_mask >> B) & 1u));
+            if (sel) acc += P.ctr[C_BUCKET + B];
+        }
+        bstart[kNumBuckets] = acc;
+        if (blockIdx.x == 0) atomicAdd(&P.stat[S_GUIDE_Q], (unsigned long long)acc);
+    }
+    // guidable types the mask leaves out: unguided
+    for (int B = 0; B < kNumMatTypes; B++) {
+        if (!guidable(B) || ((P.guide_mask >> B) & 1u)) continue;
+        const int nb = P.ctr[C_BUCKET + B];
+        for (int k = blockIdx.x * 256 + threadIdx.x; k < nb; k += gridDim.x * 256)
+            P.gdir[P.bucket[size_t(B) * P.capacity + k]] = make_float4(0.f, 0.f, 0.f, -1.f);
+    }
+    load_lds(sfrag, ffrag, kFwdFrags * 64, sbias, fbias, kBiasFwd);  // ends with a barrier
+    const int n = bstart[kNumBuckets];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, j = lane & 15, g = lane >> 4;
     for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
         int i = base + w * 64 + lane;
         int pidg = -1;
         float f[kFeat] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         if (i < n) {
-            uint32_t pid = q[i];
+            int B = 0;
+            while (i >= bstart[B + 1]) B++;
+            uint32_t pid = P.bucket[size_t(B) * P.capacity + (i - bstart[B])];
             float4 h = P.hit[pid];
             int slot = __float_as_int(h.w);
             if (slot >= 0) {

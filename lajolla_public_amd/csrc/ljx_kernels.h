@@ -51,7 +51,7 @@ struct BatchJob {
 };
 // 64-bit statistics slots
 constexpr int S_EXT_RAYS = 0, S_SH_RAYS = 1, S_NODES = 2, S_PRIMS = 3, S_SH_NODES = 4, S_SH_PRIMS = 5,
-              S_NUM = 8;
+              S_GUIDE_Q = 6, S_NUM = 8;
 
 struct PathBufs {
     uint64_t *rng;
@@ -461,7 +461,7 @@ __device__ __forceinline__ void shade_surface(const DeviceScene &S, const PathBu
         // next-event estimation (path_tracing.h:96-207)
         double lu0 = pcg_double(st, inc), lu1 = pcg_double(st, inc);
         double lw = pcg_double(st, inc), sw = pcg_double(st, inc);
-        int lid = sample_cdf(S.light_cdf, S.num_lights, lw);
+        int lid = sample_cdf_guided(S.light_cdf, S.num_lights, lw, S.light_guide, S.light_guide_m);
         const LightRec &l = S.lights[lid];
         D3 dir_light, L;
         double G = 0, tfar, lpdf;

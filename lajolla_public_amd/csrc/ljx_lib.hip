@@ -788,6 +788,15 @@ void build_scene(Scene &sc, const ljx_scene_desc *d) {
     S.tex_data = sc.upload(tex_data);
     S.light_pmf = sc.upload(lpmf);
     S.light_cdf = sc.upload(lcdf);
+    {
+        // guide table of the light CDF: guide[k] = first index with cdf > k / m
+        int m = 1;
+        while (m < d->num_lights && m < 4096) m <<= 1;
+        std::vector<int> guide(size_t(m) + 1, 0);
+        if (!lcdf.empty()) cdf_guide(lcdf.data(), int(lcdf.size()), m, guide.data());
+        S.light_guide = sc.upload(guide);
+        S.light_guide_m = m;
+    }
     S.tri_cdf = sc.upload(tri_cdf);
     S.num_lights = d->num_lights;
     S.envmap_light_id = d->envmap_light_id;
@@ -1358,6 +1367,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
         stats->train_samples = train_samples;
         stats->train_steps = train_steps;
         stats->train_step_samples = train_step_samples;
+        stats->guide_queries = st[S_GUIDE_Q];
         stats->node_visits = st[S_NODES];
         stats->tri_tests = st[S_PRIMS];
         stats->shadow_node_visits = st[S_SH_NODES];

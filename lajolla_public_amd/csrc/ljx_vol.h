@@ -228,7 +228,7 @@ __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &
         double lu0 = pcg_double(nst, ninc), lu1 = pcg_double(nst, ninc);
         double lw = pcg_double(nst, ninc), sw = pcg_double(nst, ninc);
         if (S.num_lights > 0) {
-            int lid = sample_cdf(S.light_cdf, S.num_lights, lw);
+            int lid = sample_cdf_guided(S.light_cdf, S.num_lights, lw, S.light_guide, S.light_guide_m);
             const LightRec &l = S.lights[lid];
             LightPoint lp = sample_light_point(S, l, pos, lu0, lu1, sw);
             D3 dir_light = normalize(lp.pos - pos);
