@@ -348,7 +348,65 @@ __global__ __launch_bounds__(256) void k_vol_shade(DeviceScene S, PathBufs P, in
     }
 }
 
-// The NEE shadow walk with ratio tracking (homework2.tex:770-812).
+// NEE segments, first pass: one any-hit query per segment that passes
+// through index-matched boundaries.  A surface with a material anywhere on
+// the way to the light blocks the sample whatever the media, so this
+// settles most segments before the FP64 walk: sh_occ = 1 when such a surface
+// blocks the segment, 2 when it crosses index-matched boundaries only (the
+// walk takes closest hits through them), 0 when nothing lies on it (the
+// walk's first closest-hit query, over the same interval, would find
+// nothing).  Traversal only, so it runs at the any-hit kernel's occupancy
+// and with its lane refill; the walk's FP64 state would cut that to two
+// waves per SIMD.
+template <bool STATS, bool SPH>
+__global__ __launch_bounds__(256) void k_vol_occlude(DeviceScene S, PathBufs P) {
+    __shared__ int stack[kStackLds * 256];
+    const int n = P.ctr[C_SH];
+    const TraceScene ts{S.nodes, S.prims, S.spheres, S.slot_passthru};
+    const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    int *spill = P.spill + gtid;
+    uint32_t nn = 0, np = 0;
+    unsigned long long segs = 0;
+    Tracer<true, STATS, SPH, true> tr;
+    bool busy = false, fin = false;
+    int i = 0, next = gtid;
+    while (true) {
+        const int nidle = __popcll(__ballot(!busy));
+        if (nidle >= kShadowRefill) {
+            if (fin) P.sh_occ[i] = tr.found ? 1 : (tr.passed ? 2 : 0);
+            fin = false;
+            if (!busy && next < n) {
+                i = next;
+                next += stride;
+                const float4 so = P.sh_o[i], sd = P.sh_d[i];
+                const D3 p{so.x, so.y, so.z}, pp{sd.x, sd.y, sd.z};
+                const D3 dir_light = normalize(pp - p);
+                const double dl = sqrt(dist2(p, pp));
+                tr.start(float(p.x), float(p.y), float(p.z), float(dir_light.x), float(dir_light.y),
+                         float(dir_light.z), float(S.eps), float((1 - S.eps) * dl));
+                busy = true;
+                if (S.num_nodes > 0) segs++;
+            }
+            if (__ballot(busy) == 0) break;
+        }
+        if (busy) {
+            if (S.num_nodes == 0 || tr.step(ts, stack + threadIdx.x, blockDim.x, spill, stride, nn, np)) {
+                busy = false;
+                fin = true;
+            }
+        }
+    }
+    atomicAdd(&P.stat[S_SH_RAYS], segs);
+    if (STATS) {
+        atomicAdd(&P.stat[S_SH_NODES], (unsigned long long)nn);
+        atomicAdd(&P.stat[S_SH_PRIMS], (unsigned long long)np);
+    }
+}
+
+// The NEE shadow walk with ratio tracking (homework2.tex:770-812), after
+// k_vol_occlude: blocked segments are skipped, and only segments that cross
+// index-matched boundaries take closest hits.
 template <bool STATS, bool SPH>
 __global__ __launch_bounds__(256) void k_vol_shadow(DeviceScene S, PathBufs P) {
     __shared__ int stack[kStackLds * 256];
@@ -375,24 +433,13 @@ __global__ __launch_bounds__(256) void k_vol_shadow(DeviceScene S, PathBufs P) {
         D3 p0 = p;
         bool blocked = false;
         int shadow_bounces = 0;
-        // a surface with a material anywhere on the way to the light blocks
-        // the sample whatever the media: one any-hit query that passes
-        // through index-matched boundaries settles that before the walk (the
-        // NEE stream is not written back, so skipping the walk leaves every
-        // other draw unchanged); if it met no boundary either, the walk's
-        // first closest-hit query, over the same interval, finds nothing
+        // k_vol_occlude's verdict (the NEE stream is not written back, so
+        // skipping a blocked segment's walk leaves every other draw unchanged)
         bool crossings = false;
         if (S.num_nodes > 0) {
-            const double dl = sqrt(dist2(p, pp));
-            Tracer<true, STATS, SPH, true> tr;
-            tr.start(float(p.x), float(p.y), float(p.z), float(dir_light.x), float(dir_light.y), float(dir_light.z),
-                     float(S.eps), float((1 - S.eps) * dl));
-            const TraceScene ts{S.nodes, S.prims, S.spheres, S.slot_passthru};
-            while (!tr.step(ts, stack + threadIdx.x, blockDim.x, spill, stride, nn, np)) {
-            }
-            segs++;
-            if (tr.found) continue;
-            crossings = tr.passed;
+            const uint8_t occ = P.sh_occ[i];
+            if (occ == 1) continue;
+            crossings = occ == 2;
         }
         for (int seg = 0; seg < kMaxShadowSegments; seg++) {
             double dl = sqrt(dist2(p0, pp));
@@ -517,6 +564,14 @@ inline void vol_shade_all(const DeviceScene &S, PathBufs &P, int cur, int grid, 
 }
 
 inline void vol_shadow(const DeviceScene &S, PathBufs &P, int grid, hipStream_t s, bool stats, bool sph) {
+    if (stats) {
+        if (sph) hipLaunchKernelGGL((k_vol_occlude<true, true>), dim3(grid), dim3(256), 0, s, S, P);
+        else hipLaunchKernelGGL((k_vol_occlude<true, false>), dim3(grid), dim3(256), 0, s, S, P);
+    } else {
+        if (sph) hipLaunchKernelGGL((k_vol_occlude<false, true>), dim3(grid), dim3(256), 0, s, S, P);
+        else hipLaunchKernelGGL((k_vol_occlude<false, false>), dim3(grid), dim3(256), 0, s, S, P);
+    }
+    vol_launch_check();
     if (stats) {
         if (sph) hipLaunchKernelGGL((k_vol_shadow<true, true>), dim3(grid), dim3(256), 0, s, S, P);
         else hipLaunchKernelGGL((k_vol_shadow<true, false>), dim3(grid), dim3(256), 0, s, S, P);
