@@ -771,9 +771,23 @@ LJX_HD BsdfSample disney_bsdf_sample(const MatRec &m, D3 in, const ShadeGeom &g,
 // MT is the material type tag of include/lajolla_amd.h; MT = -1 dispatches
 // at run time.  Shading kernels are instantiated per type so that each
 // sorted bucket runs straight-line code.
+// The material type a BSDF call switches on: MT itself in a kernel of one
+// type; the record's own type under runtime dispatch (MT = -1); and under
+// runtime dispatch over lajolla's three basic materials only (MT =
+// kMtBasic: Lambertian, RoughPlastic, RoughDielectric; the host picks it
+// only for scenes that use no other type) the type clamped to them, so the
+// Disney lobes drop out of the kernel and its register budget.
+constexpr int kMtBasic = -2;
+template <int MT>
+LJX_HD int dispatch_type(const MatRec &m) {
+    if constexpr (MT >= 0) return MT;
+    else if constexpr (MT == kMtBasic) return m.type < 2 ? m.type : 2;
+    else return m.type;
+}
+
 template <int MT>
 LJX_HD D3 bsdf_eval(const MatRec &m, D3 in, D3 out, const ShadeGeom &g, bool to_view = false) {
-    const int t = MT >= 0 ? MT : m.type;
+    const int t = dispatch_type<MT>(m);
     switch (t) {
     case 0: return lambert_eval(m.a, in, out, g);
     case 1: return plastic_eval(m, in, out, g);
@@ -792,7 +806,7 @@ LJX_HD D3 bsdf_eval(const MatRec &m, D3 in, D3 out, const ShadeGeom &g, bool to_
 }
 template <int MT>
 LJX_HD double bsdf_pdf(const MatRec &m, D3 in, D3 out, const ShadeGeom &g) {
-    const int t = MT >= 0 ? MT : m.type;
+    const int t = dispatch_type<MT>(m);
     switch (t) {
     case 0: return cos_pdf(in, out, g);
     case 1: return plastic_pdf(m, in, out, g);
@@ -811,7 +825,7 @@ LJX_HD double bsdf_pdf(const MatRec &m, D3 in, D3 out, const ShadeGeom &g) {
 }
 template <int MT>
 LJX_HD BsdfSample bsdf_sample(const MatRec &m, D3 in, const ShadeGeom &g, double u0, double u1, double w) {
-    const int t = MT >= 0 ? MT : m.type;
+    const int t = dispatch_type<MT>(m);
     switch (t) {
     case 0: return cos_sample(in, g, u0, u1, 1.0);
     case 1: return plastic_sample(m, in, g, u0, u1, w);

@@ -1305,7 +1305,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
                     tm.end(4, ev);
                 }
                 tm.begin(2, ev);
-                if (vol) vol_shade_all(S, P, cur, shade_grid, s, sc.has_spheres);
+                if (vol) vol_shade_all(S, P, cur, shade_grid, s, sc.has_spheres, (sc.mat_types_present & ~7u) == 0);
                 else shade_all(sc, P, cur, shade_grid, s, guided, shade_block);
                 build_ext_queue(P, ns * nq, cur, s);
                 tm.end(2, ev);
@@ -1610,7 +1610,7 @@ void batch_impl(const ljx_batch_job *jobs, size_t njobs, const ljx_exec_options 
                 int cur = 0;
                 for (int b = 0; b < std::max(P.max_depth, 1); b++) {
                     trace_stage(sc, P, cur, tgrid, sgrid, cs, false, false);
-                    if (vol) vol_shade_all(S, P, cur, sgrid, cs, sc.has_spheres);
+                    if (vol) vol_shade_all(S, P, cur, sgrid, cs, sc.has_spheres, (sc.mat_types_present & ~7u) == 0);
                     else shade_all(sc, P, cur, sgrid, cs, false, gp.block);
                     build_ext_queue(P, n, cur, cs);
                     if (vol) vol_shadow(S, P, tgrid, cs, false, sc.has_spheres);

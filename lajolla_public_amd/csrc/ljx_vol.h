@@ -136,7 +136,8 @@ __global__ __launch_bounds__(256) void k_vol_flight(DeviceScene S, PathBufs P) {
     }
 }
 
-template <bool SPH>
+// DMT: the BSDF dispatch, -1 (every material type) or kMtBasic.
+template <bool SPH, int DMT>
 __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &P, uint32_t pid, bool &push_ext,
                                           bool &push_sh, float4 &sh_o, float4 &sh_d, float4 &sh_c, float4 &sh_x) {
     float4 h = P.hit[pid];
@@ -247,8 +248,8 @@ __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &
                 } else {
                     MatRec mat = S.materials[v.material_id];
                     if (mat.textured) resolve_textures(S, mat, v.tu, v.tv, 0.0);
-                    f = bsdf_eval<-1>(mat, dir_view, dir_light, v.g);
-                    pdf_f = bsdf_pdf<-1>(mat, dir_view, dir_light, v.g);
+                    f = bsdf_eval<DMT>(mat, dir_view, dir_light, v.g);
+                    pdf_f = bsdf_pdf<DMT>(mat, dir_view, dir_light, v.g);
                 }
                 D3 L = light_emission(S, l, -dir_light, lp);
                 D3 K = pre * G * f * L;
@@ -281,13 +282,13 @@ __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &
             MatRec mat = S.materials[v.material_id];
             if (mat.textured) resolve_textures(S, mat, v.tu, v.tv, 0.0);
             double u0 = pcg_double(st, inc), u1 = pcg_double(st, inc), w = pcg_double(st, inc);
-            BsdfSample bs = bsdf_sample<-1>(mat, dir_view, v.g, u0, u1, w);
+            BsdfSample bs = bsdf_sample<DMT>(mat, dir_view, v.g, u0, u1, w);
             if (!bs.valid) {
                 alive = false;
             } else {
                 if (bs.eta != 0) eta_scale /= (bs.eta * bs.eta);
-                D3 f = bsdf_eval<-1>(mat, dir_view, bs.dir, v.g);
-                double pdf = bsdf_pdf<-1>(mat, dir_view, bs.dir, v.g);
+                D3 f = bsdf_eval<DMT>(mat, dir_view, bs.dir, v.g);
+                double pdf = bsdf_pdf<DMT>(mat, dir_view, bs.dir, v.g);
                 if (pdf <= 0) {
                     alive = false;
                 } else {
@@ -322,7 +323,7 @@ __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &
                              never_scatter ? 1.f : 0.f);
 }
 
-template <bool SPH>
+template <bool SPH, int DMT>
 __global__ __launch_bounds__(256) void k_vol_shade(DeviceScene S, PathBufs P, int cur) {
     int n = P.ctr[C_BUCKET + 0];
     const uint32_t *q = P.bucket;
@@ -334,7 +335,7 @@ __global__ __launch_bounds__(256) void k_vol_shade(DeviceScene S, PathBufs P, in
         uint32_t pid = 0;
         if (i < n) {
             pid = q[i];
-            vol_shade<SPH>(S, P, pid, pe, ps, so, sd, sc, sx);
+            vol_shade<SPH, DMT>(S, P, pid, pe, ps, so, sd, sc, sx);
         }
         if (pe) P.alive[pid] = 1;
         int j = wave_append(ps, &P.ctr[C_SH]);
@@ -552,14 +553,20 @@ inline void vol_launch_check() {
     if (e != hipSuccess) throw std::runtime_error(std::string("volpath kernel launch: ") + hipGetErrorString(e));
 }
 
-inline void vol_shade_all(const DeviceScene &S, PathBufs &P, int cur, int grid, hipStream_t s, bool sph) {
+// basic: the scene uses no material types beyond lajolla's three basic ones.
+inline void vol_shade_all(const DeviceScene &S, PathBufs &P, int cur, int grid, hipStream_t s, bool sph, bool basic) {
     if (P.vol_flight) {
         if (sph) hipLaunchKernelGGL(k_vol_flight<true>, dim3(grid), dim3(256), 0, s, S, P);
         else hipLaunchKernelGGL(k_vol_flight<false>, dim3(grid), dim3(256), 0, s, S, P);
         vol_launch_check();
     }
-    if (sph) hipLaunchKernelGGL(k_vol_shade<true>, dim3(grid), dim3(256), 0, s, S, P, cur);
-    else hipLaunchKernelGGL(k_vol_shade<false>, dim3(grid), dim3(256), 0, s, S, P, cur);
+    if (basic) {
+        if (sph) hipLaunchKernelGGL((k_vol_shade<true, kMtBasic>), dim3(grid), dim3(256), 0, s, S, P, cur);
+        else hipLaunchKernelGGL((k_vol_shade<false, kMtBasic>), dim3(grid), dim3(256), 0, s, S, P, cur);
+    } else {
+        if (sph) hipLaunchKernelGGL((k_vol_shade<true, -1>), dim3(grid), dim3(256), 0, s, S, P, cur);
+        else hipLaunchKernelGGL((k_vol_shade<false, -1>), dim3(grid), dim3(256), 0, s, S, P, cur);
+    }
     vol_launch_check();
 }
 

@@ -134,3 +134,21 @@ def test_analytic_glass_sphere():
     print(f"z>4 fraction {(z > 4).mean():.4f}; means {G.mean():.5f} {O.mean():.5f}", flush=True)
     assert (z > 4).mean() < 0.01
     assert abs(G.mean() - O.mean()) < 0.01 * O.mean()
+
+
+@pytest.mark.parametrize("mat", ["basic", "disney"])
+def test_volpath_material_dispatch_sample_matched(mat):
+    """Volumetric shading dispatches BSDFs over lajolla's three basic
+    materials only when the scene uses no others, else over every type:
+    the nested-media scene as it is (Lambertian, RoughDielectric) and with
+    its floor made DisneyBSDF, both sample-matched to the oracle."""
+    desc = scenes.nested_media(seed=11, g=0.7, width=40, height=40, spp=8, max_depth=10, sphere_res=32)
+    if mat == "disney":
+        floor = [s for s in desc.shapes if s["positions"].shape[0] == 4 and s["positions"][:, 1].max() == 0.0][0]
+        floor["material_id"] = desc.add_material(capi.MAT_DISNEY_BSDF, (0.6, 0.5, 0.4), roughness=0.4, metallic=0.3,
+                                                 sheen=0.5, clearcoat=0.4, clearcoat_gloss=0.6)
+    g = api.Scene(desc).render()
+    o, _ = po.OracleScene(desc).render(plan=1)
+    assert np.isfinite(g).all()
+    sample_matched(g, o, frac=0.90)
+    assert abs(g.mean() - o.mean()) < 0.03 * o.mean()
