@@ -99,13 +99,18 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
 }
 __device__ __forceinline__ f32x4 zero4() { return f32x4{0.f, 0.f, 0.f, 0.f}; }
 
+// Bin e of the group: exp(-(e - c)^2 / 2) = 2^(e c L - c^2 L / 2 - e^2 L / 2)
+// with L = log2(e): one add to the running exponent and one exp2 per bin.
 __device__ __forceinline__ bf16x8 encode8(float s, int nbins, int bin0) {
-    float c = s * float(nbins) - float(bin0) - 0.5f;
+    constexpr float L = 1.4426950408889634f;
+    const float c = s * float(nbins) - float(bin0) - 0.5f;
+    const float cl = c * L;
+    float q = -0.5f * L * c * c;  // exponent of bin 0
     bf16x8 r;
 #pragma unroll
     for (int e = 0; e < 8; e++) {
-        float d = float(e) - c;
-        r[e] = (__bf16)__expf(-0.5f * d * d);
+        r[e] = (__bf16)__builtin_amdgcn_exp2f(q);
+        q += cl - L * (float(e) + 0.5f);  // exponent of bin e + 1
     }
     return r;
 }
@@ -124,11 +129,18 @@ __device__ __forceinline__ void mlp_forward(const float *feat, const bf16x8 *fra
     const int j = lane & 15, g = lane >> 4;
     f32x4 acc[NS][4];
     bf16x8 hb[NS][2];
+    // every layer's accumulators start from its bias (rows 16 t + 4 g + e)
+    auto bias4 = [&](int l, int t) {
+        const float *b = bias + bias_base(l) + 16 * t + 4 * g;
+        return f32x4{b[0], b[1], b[2], b[3]};
+    };
     // layer 0: one-blob input computed per K block
 #pragma unroll
-    for (int s = 0; s < NS; s++)
+    for (int t = 0; t < 4; t++) {
+        const f32x4 b = bias4(0, t);
 #pragma unroll
-        for (int t = 0; t < 4; t++) acc[s][t] = zero4();
+        for (int s = 0; s < NS; s++) acc[s][t] = b;
+    }
 #pragma unroll
     for (int kb = 0; kb < 5; kb++) {
         int f, nb, b0;
@@ -146,25 +158,24 @@ __device__ __forceinline__ void mlp_forward(const float *feat, const bf16x8 *fra
     // hidden layers 1..3 and the output layer
 #pragma unroll
     for (int l = 1; l <= 4; l++) {
-        // bias + ReLU of layer l-1, packed as layer l's B operand
+        // ReLU of layer l-1 (its bias is in the accumulators), packed as
+        // layer l's B operand
 #pragma unroll
         for (int s = 0; s < NS; s++) {
 #pragma unroll
             for (int kb = 0; kb < 2; kb++) {
 #pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    int t = 2 * kb + (e >> 2);
-                    float v = acc[s][t][e & 3] + bias[bias_base(l - 1) + 16 * t + 4 * g + (e & 3)];
-                    hb[s][kb][e] = (__bf16)fmaxf(v, 0.f);
-                }
+                for (int e = 0; e < 8; e++) hb[s][kb][e] = (__bf16)fmaxf(acc[s][2 * kb + (e >> 2)][e & 3], 0.f);
                 if (H) H[l - 1][s][kb] = hb[s][kb];
             }
         }
         const int T = l == 4 ? 3 : 4;
 #pragma unroll
-        for (int s = 0; s < NS; s++)
+        for (int t = 0; t < T; t++) {
+            const f32x4 b = bias4(l, t);
 #pragma unroll
-            for (int t = 0; t < 4; t++) acc[s][t] = zero4();
+            for (int s = 0; s < NS; s++) acc[s][t] = b;
+        }
 #pragma unroll
         for (int kb = 0; kb < 2; kb++) {
 #pragma unroll
@@ -178,9 +189,7 @@ __device__ __forceinline__ void mlp_forward(const float *feat, const bf16x8 *fra
 #pragma unroll
     for (int s = 0; s < NS; s++)
 #pragma unroll
-        for (int t = 0; t < 3; t++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) out[s][t][e] = acc[s][t][e] + bias[bias_base(4) + 16 * t + 4 * g + e];
+        for (int t = 0; t < 3; t++) out[s][t] = acc[s][t];
 }
 
 // Raw parameters of the lane's two lobes from its 12 output slots.
@@ -216,10 +225,10 @@ __device__ __forceinline__ float log_vmf_norm(float kappa) {
 }
 __device__ __forceinline__ LobeEval eval_lobe(const Lobe &L, float log_z) {
     LobeEval e;
-    float len = sqrtf(fmaxf(L.rx * L.rx + L.ry * L.ry + L.rz * L.rz, 1e-12f));
-    e.mx = L.rx / len;
-    e.my = L.ry / len;
-    e.mz = L.rz / len;
+    const float il = rsqrtf(fmaxf(L.rx * L.rx + L.ry * L.ry + L.rz * L.rz, 1e-12f));
+    e.mx = L.rx * il;
+    e.my = L.ry * il;
+    e.mz = L.rz * il;
     e.kappa = __expf(fminf(fmaxf(L.rk, kLogKappaLo), kLogKappaHi));
     e.logw = L.logit - log_z;
     e.kmx = e.kappa * e.mx;

@@ -1118,10 +1118,10 @@ void ensure_guide(Scene &sc, RenderCtx &rc, int rec_depth) {
 }
 
 void guide_infer(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s, const guide::Net &net) {
-    if (sc.has_spheres) launch(k_guide_infer<true>, grid, 256, s, sc.ds, P, cur, (const guide::bf16x8 *)net.ffrag,
-                               (const float *)net.fbias);
-    else launch(k_guide_infer<false>, grid, 256, s, sc.ds, P, cur, (const guide::bf16x8 *)net.ffrag,
-                (const float *)net.fbias);
+    const auto *ff = (const guide::bf16x8 *)net.ffrag;
+    const auto *fb = (const float *)net.fbias;
+    if (sc.has_spheres) launch(k_guide_infer<true>, grid, 512, s, sc.ds, P, cur, ff, fb);
+    else launch(k_guide_infer<false>, grid, 512, s, sc.ds, P, cur, ff, fb);
 }
 
 // The auxiliary integrators: one launch over the pixels, no accumulation.
@@ -1249,7 +1249,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
     }
     const DeviceScene &S = sc.ds;
     int shade_grid = std::max(1, std::min(capacity / shade_block + 1, sc.num_cu * 16 * 256 / shade_block));
-    int guide_grid = std::max(1, std::min(capacity / 256 + 1, sc.num_cu * 2));
+    int guide_grid = std::max(1, std::min(capacity / 512 + 1, sc.num_cu * 2));
     int tgrid = rc.trace_grid;
     Timer tm{e.collect_timing != 0, s};
     bool stats_on = e.collect_traversal != 0;
