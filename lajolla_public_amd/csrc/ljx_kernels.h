@@ -36,6 +36,7 @@ constexpr int C_EXT = 0;      // C_EXT + (0|1): extension queues
 constexpr int C_SH = 2;       // shadow queue
 constexpr int C_BUCKET = 4;   // C_BUCKET + bucket
 constexpr int C_XFETCH = 16;  // C_XFETCH + (0|1): rays taken from extension queue 0|1 by the traversal
+constexpr int C_FLQ = 18;     // volumetric free-flight queue: its length, then (C_FLQ + 1) paths taken from it
 constexpr int C_TOTAL = 16 + 4;
 
 // One job of a batched render: its camera, image and place in the batch's
@@ -76,6 +77,7 @@ struct PathBufs {
     float4 *sh_x;     // shadow walk: pdf_nee, pdf_dir, NEE stream state (lo, hi bits)
     double4 *vf_t;    // free flight: transmittance rgb, distance travelled
     double4 *vf_p;    // free flight: transmittance pdf rgb, 1 if it ended in a real collision
+    uint32_t *flq;    // free flight: the paths of the volumetric queue that are in a medium
     int *ctr;
     unsigned long long *stat;
     int *spill;

@@ -66,42 +66,67 @@ struct Flight {
     double accum_t;
     bool scatter;
 };
+// The loop of free_flight, resumable one collision at a time.
+struct FlightRun {
+    const MediumRec *m;
+    D3 org, dir, maj;
+    double t_hit, max_maj;
+    int ch, it;
+    Flight fl;
+};
+__device__ __forceinline__ void flight_begin(FlightRun &r, const DeviceScene &S, int medium, D3 org, D3 dir,
+                                             double t_hit, uint64_t &st, uint64_t inc) {
+    r.fl = Flight{{1, 1, 1}, {1, 1, 1}, 0.0, false};
+    r.m = &S.media[medium];
+    r.org = org;
+    r.dir = dir;
+    r.t_hit = t_hit;
+    r.maj = medium_majorant(*r.m, org, dir, __builtin_huge_val());
+    r.max_maj = maxc(r.maj);
+    const double u = pcg_double(st, inc);
+    r.ch = min(max(int(u * 3), 0), 2);
+    r.it = 0;
+}
+// One pass of the loop; true when the flight has ended.
+__device__ __forceinline__ bool flight_step(FlightRun &r, const PathBufs &P, uint64_t &st, uint64_t inc) {
+    Flight &fl = r.fl;
+    const D3 maj = r.maj;
+    const double max_maj = r.max_maj;
+    const int ch = r.ch;
+    const double mc = comp(maj, ch);
+    if (mc <= 0) return true;
+    if (r.it >= P.max_null) return true;
+    const double t = -log(1 - pcg_double(st, inc)) / mc;
+    const double dt = r.t_hit - fl.accum_t;
+    fl.accum_t = fmin(fl.accum_t + t, r.t_hit);
+    if (t < dt) {
+        D3 sig_t = sigma_t_at(*r.m, r.org + fl.accum_t * r.dir);
+        D3 real_prob = sig_t / maj;
+        D3 e = vexp(-1.0 * maj * t);
+        if (pcg_double(st, inc) < comp(real_prob, ch)) {
+            fl.scatter = true;
+            fl.T = fl.T * e / max_maj;
+            fl.trans_dir_pdf = fl.trans_dir_pdf * e * maj * real_prob / max_maj;
+            return true;
+        }
+        D3 sig_n = maj - sig_t;
+        fl.T = fl.T * e * sig_n / max_maj;
+        fl.trans_dir_pdf = fl.trans_dir_pdf * e * maj * (1.0 - real_prob) / max_maj;
+        r.it++;
+        return false;
+    }
+    D3 e = vexp(-1.0 * maj * dt);
+    fl.T = fl.T * e;
+    fl.trans_dir_pdf = fl.trans_dir_pdf * e;
+    return true;
+}
 __device__ __forceinline__ Flight free_flight(const DeviceScene &S, const PathBufs &P, int medium, D3 org, D3 dir,
                                               double t_hit, uint64_t &st, uint64_t inc) {
-    Flight fl{{1, 1, 1}, {1, 1, 1}, 0.0, false};
-    const MediumRec &m = S.media[medium];
-    D3 maj = medium_majorant(m, org, dir, __builtin_huge_val());
-    double max_maj = maxc(maj);
-    double u = pcg_double(st, inc);
-    int ch = min(max(int(u * 3), 0), 2);
-    for (int it = 0;; it++) {
-        double mc = comp(maj, ch);
-        if (mc <= 0) break;
-        if (it >= P.max_null) break;
-        double t = -log(1 - pcg_double(st, inc)) / mc;
-        double dt = t_hit - fl.accum_t;
-        fl.accum_t = fmin(fl.accum_t + t, t_hit);
-        if (t < dt) {
-            D3 sig_t = sigma_t_at(m, org + fl.accum_t * dir);
-            D3 real_prob = sig_t / maj;
-            D3 e = vexp(-1.0 * maj * t);
-            if (pcg_double(st, inc) < comp(real_prob, ch)) {
-                fl.scatter = true;
-                fl.T = fl.T * e / max_maj;
-                fl.trans_dir_pdf = fl.trans_dir_pdf * e * maj * real_prob / max_maj;
-                break;
-            }
-            D3 sig_n = maj - sig_t;
-            fl.T = fl.T * e * sig_n / max_maj;
-            fl.trans_dir_pdf = fl.trans_dir_pdf * e * maj * (1.0 - real_prob) / max_maj;
-        } else {
-            D3 e = vexp(-1.0 * maj * dt);
-            fl.T = fl.T * e;
-            fl.trans_dir_pdf = fl.trans_dir_pdf * e;
-            break;
-        }
+    FlightRun r;
+    flight_begin(r, S, medium, org, dir, t_hit, st, inc);
+    while (!flight_step(r, P, st, inc)) {
     }
-    return fl;
+    return r.fl;
 }
 
 // Distance to the surface a ray hit (infinite on a miss), from the vertex
@@ -114,29 +139,74 @@ __device__ __forceinline__ double hit_distance(const DeviceScene &S, float4 h, D
     return sqrt(dist2(org, v.pos));
 }
 
-// Free flight for every path of the volumetric queue that is in a medium;
-// lanes run their paths one after another (the loop lengths vary widely) in a
-// kernel small enough for full occupancy.
-template <bool SPH>
-__global__ __launch_bounds__(256) void k_vol_flight(DeviceScene S, PathBufs P) {
+// The paths of the volumetric queue that are in a medium, for k_vol_flight
+// (the order does not matter: each path's flight uses its own stream).
+__global__ __launch_bounds__(256) void k_vol_flight_queue(PathBufs P) {
     const int n = P.ctr[C_BUCKET + 0];
     const int stride = gridDim.x * blockDim.x;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t pid = P.bucket[i];
-        const int medium = P.medium[pid];
-        if (medium < 0) continue;
-        const float4 ro = P.ray_o[pid], rd = P.ray_d[pid];
-        const D3 org = f3d(ro), dir = f3d(rd);
-        uint64_t st = P.rng[pid];
-        const uint64_t inc = pcg_inc(slot_info(P, int(pid)).stream());
-        const Flight fl = free_flight(S, P, medium, org, dir, hit_distance<SPH>(S, P.hit[pid], org, dir), st, inc);
-        P.rng[pid] = st;
-        P.vf_t[pid] = make_double4(fl.T.x, fl.T.y, fl.T.z, fl.accum_t);
-        P.vf_p[pid] = make_double4(fl.trans_dir_pdf.x, fl.trans_dir_pdf.y, fl.trans_dir_pdf.z, fl.scatter ? 1.0 : 0.0);
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const int i = base + threadIdx.x;
+        uint32_t pid = 0;
+        bool in_medium = false;
+        if (i < n) {
+            pid = P.bucket[i];
+            in_medium = P.medium[pid] >= 0;
+        }
+        const int k = wave_append(in_medium, &P.ctr[C_FLQ]);
+        if (k >= 0) P.flq[k] = pid;
     }
 }
 
-// DMT: the BSDF dispatch, -1 (every material type) or kMtBasic.
+// Free flight for every path of the flight queue, in a kernel small enough
+// for three waves per SIMD.  The loops' lengths vary widely between paths, so
+// a lane whose flight ended takes its next path as soon as kFlightRefill
+// lanes of its wave are idle (one atomic per wave for all of them) rather
+// than waiting for the wave's longest flight.
+constexpr int kFlightRefill = 16;
+template <bool SPH>
+__global__ __launch_bounds__(256) void k_vol_flight(DeviceScene S, PathBufs P) {
+    const int n = P.ctr[C_FLQ];
+    int *fetch = &P.ctr[C_FLQ + 1];
+    const int lane = __lane_id();
+    FlightRun r;
+    uint64_t st = 0, inc = 0;
+    uint32_t pid = 0;
+    bool busy = false, fin = false, more = true;  // more: wave-uniform
+    while (true) {
+        const uint64_t idle = __ballot(!busy);
+        const int nidle = __popcll(idle);
+        if ((more && nidle >= kFlightRefill) || nidle == 64) {
+            if (fin) {
+                P.rng[pid] = st;
+                P.vf_t[pid] = make_double4(r.fl.T.x, r.fl.T.y, r.fl.T.z, r.fl.accum_t);
+                P.vf_p[pid] = make_double4(r.fl.trans_dir_pdf.x, r.fl.trans_dir_pdf.y, r.fl.trans_dir_pdf.z,
+                                           r.fl.scatter ? 1.0 : 0.0);
+                fin = false;
+            }
+            if (!more) break;  // every lane idle and the queue drained
+            int base = 0;
+            if (lane == 0) base = atomicAdd(fetch, nidle);
+            base = __shfl(base, 0);
+            if (base + nidle >= n) more = false;
+            const int i = base + int(__builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0)));
+            if (!busy && i < n) {
+                pid = P.flq[i];
+                const float4 ro = P.ray_o[pid], rd = P.ray_d[pid];
+                const D3 org = f3d(ro), dir = f3d(rd);
+                st = P.rng[pid];
+                inc = pcg_inc(slot_info(P, int(pid)).stream());
+                flight_begin(r, S, P.medium[pid], org, dir, hit_distance<SPH>(S, P.hit[pid], org, dir), st, inc);
+                busy = true;
+            }
+        }
+        if (busy && flight_step(r, P, st, inc)) {
+            busy = false;
+            fin = true;
+        }
+    }
+}
+
 template <bool SPH, int DMT>
 __device__ __forceinline__ void vol_shade(const DeviceScene &S, const PathBufs &P, uint32_t pid, bool &push_ext,
                                           bool &push_sh, float4 &sh_o, float4 &sh_d, float4 &sh_c, float4 &sh_x) {
@@ -546,6 +616,7 @@ void vol_alloc(bool, size_t capacity, PathBufs &P, Bufs &bufs, Alloc alloc) {
     P.sh_x = static_cast<float4 *>(take(capacity * sizeof(float4)));
     P.vf_t = static_cast<double4 *>(take(capacity * sizeof(double4)));
     P.vf_p = static_cast<double4 *>(take(capacity * sizeof(double4)));
+    P.flq = static_cast<uint32_t *>(take(capacity * sizeof(uint32_t)));
 }
 
 inline void vol_launch_check() {
@@ -556,6 +627,8 @@ inline void vol_launch_check() {
 // basic: the scene uses no material types beyond lajolla's three basic ones.
 inline void vol_shade_all(const DeviceScene &S, PathBufs &P, int cur, int grid, hipStream_t s, bool sph, bool basic) {
     if (P.vol_flight) {
+        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, P.ctr, C_FLQ, 2);
+        hipLaunchKernelGGL(k_vol_flight_queue, dim3(grid), dim3(256), 0, s, P);
         if (sph) hipLaunchKernelGGL(k_vol_flight<true>, dim3(grid), dim3(256), 0, s, S, P);
         else hipLaunchKernelGGL(k_vol_flight<false>, dim3(grid), dim3(256), 0, s, S, P);
         vol_launch_check();
