@@ -48,6 +48,17 @@ __device__ __forceinline__ D3 sigma_s_at(const MediumRec &m, D3 p) {
     return s;
 }
 
+// exp(-maj t) per channel.  A grey majorant (every channel equal: grid
+// media, whose sigma_t is the density alone) gives three identical
+// exponentials, computed once.
+__device__ __forceinline__ D3 maj_exp(D3 maj, bool grey, double t) {
+    if (grey) {
+        const double e = exp(-1.0 * maj.x * t);
+        return D3{e, e, e};
+    }
+    return vexp(-1.0 * maj * t);
+}
+
 // update_medium (homework2.tex:402-409)
 __device__ __forceinline__ int update_medium(D3 dir, const Vertex &v, int medium) {
     if (v.interior_medium_id != v.exterior_medium_id)
@@ -72,6 +83,7 @@ struct FlightRun {
     D3 org, dir, maj;
     double t_hit, max_maj;
     int ch, it;
+    bool grey;
     Flight fl;
 };
 __device__ __forceinline__ void flight_begin(FlightRun &r, const DeviceScene &S, int medium, D3 org, D3 dir,
@@ -83,6 +95,7 @@ __device__ __forceinline__ void flight_begin(FlightRun &r, const DeviceScene &S,
     r.t_hit = t_hit;
     r.maj = medium_majorant(*r.m, org, dir, __builtin_huge_val());
     r.max_maj = maxc(r.maj);
+    r.grey = r.maj.x == r.maj.y && r.maj.y == r.maj.z;
     const double u = pcg_double(st, inc);
     r.ch = min(max(int(u * 3), 0), 2);
     r.it = 0;
@@ -102,7 +115,7 @@ __device__ __forceinline__ bool flight_step(FlightRun &r, const PathBufs &P, uin
     if (t < dt) {
         D3 sig_t = sigma_t_at(*r.m, r.org + fl.accum_t * r.dir);
         D3 real_prob = sig_t / maj;
-        D3 e = vexp(-1.0 * maj * t);
+        D3 e = maj_exp(maj, r.grey, t);
         if (pcg_double(st, inc) < comp(real_prob, ch)) {
             fl.scatter = true;
             fl.T = fl.T * e / max_maj;
@@ -115,7 +128,7 @@ __device__ __forceinline__ bool flight_step(FlightRun &r, const PathBufs &P, uin
         r.it++;
         return false;
     }
-    D3 e = vexp(-1.0 * maj * dt);
+    D3 e = maj_exp(maj, r.grey, dt);
     fl.T = fl.T * e;
     fl.trans_dir_pdf = fl.trans_dir_pdf * e;
     return true;
@@ -140,10 +153,15 @@ __device__ __forceinline__ double hit_distance(const DeviceScene &S, float4 h, D
 }
 
 // The paths of the volumetric queue that are in a medium, for k_vol_flight
-// (the order does not matter: each path's flight uses its own stream).
-__global__ __launch_bounds__(256) void k_vol_flight_queue(PathBufs P) {
+// (the order does not matter: each path's flight uses its own stream).  Each
+// block of 1,024 threads appends with one atomic per round (a wave per
+// atomic contended on the one counter).
+__global__ __launch_bounds__(1024) void k_vol_flight_queue(PathBufs P) {
+    __shared__ int wcount[16];
+    __shared__ int bbase;
     const int n = P.ctr[C_BUCKET + 0];
     const int stride = gridDim.x * blockDim.x;
+    const int w = threadIdx.x >> 6;
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const int i = base + threadIdx.x;
         uint32_t pid = 0;
@@ -152,8 +170,21 @@ __global__ __launch_bounds__(256) void k_vol_flight_queue(PathBufs P) {
             pid = P.bucket[i];
             in_medium = P.medium[pid] >= 0;
         }
-        const int k = wave_append(in_medium, &P.ctr[C_FLQ]);
-        if (k >= 0) P.flq[k] = pid;
+        const uint64_t mask = __ballot(in_medium);
+        if ((threadIdx.x & 63) == 0) wcount[w] = __popcll(mask);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int total = 0;
+            for (int k = 0; k < 16; k++) total += wcount[k];
+            bbase = total > 0 ? atomicAdd(&P.ctr[C_FLQ], total) : 0;
+        }
+        __syncthreads();
+        if (in_medium) {
+            int k = bbase + int(__builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0)));
+            for (int q = 0; q < w; q++) k += wcount[q];
+            P.flq[k] = pid;
+        }
+        __syncthreads();
     }
 }
 
@@ -531,6 +562,7 @@ __global__ __launch_bounds__(256) void k_vol_shadow(DeviceScene S, PathBufs P) {
                 const MediumRec &m = S.media[medium];
                 D3 maj = medium_majorant(m, p0, dir_light, (1 - S.eps) * dl);
                 double max_maj = maxc(maj);
+                const bool grey = maj.x == maj.y && maj.y == maj.z;
                 double u = pcg_double(nst, ninc);
                 int ch = min(max(int(u * 3), 0), 2);
                 double accum_t = 0;
@@ -544,14 +576,14 @@ __global__ __launch_bounds__(256) void k_vol_shadow(DeviceScene S, PathBufs P) {
                     if (t < dt) {
                         D3 sig_t = sigma_t_at(m, p0 + accum_t * dir_light);
                         D3 sig_n = maj - sig_t;
-                        D3 e = vexp(-1.0 * maj * t);
+                        D3 e = maj_exp(maj, grey, t);
                         T = T * e * sig_n / max_maj;
                         p_nee = p_nee * e * maj / max_maj;
                         D3 real_prob = sig_t / maj;
                         p_dir = p_dir * e * maj * (1.0 - real_prob) / max_maj;
                         if (maxc(T) <= 0) break;
                     } else {
-                        D3 e = vexp(-1.0 * maj * dt);
+                        D3 e = maj_exp(maj, grey, dt);
                         T = T * e;
                         p_nee = p_nee * e;
                         p_dir = p_dir * e;
@@ -628,7 +660,7 @@ inline void vol_launch_check() {
 inline void vol_shade_all(const DeviceScene &S, PathBufs &P, int cur, int grid, hipStream_t s, bool sph, bool basic) {
     if (P.vol_flight) {
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, P.ctr, C_FLQ, 2);
-        hipLaunchKernelGGL(k_vol_flight_queue, dim3(grid), dim3(256), 0, s, P);
+        hipLaunchKernelGGL(k_vol_flight_queue, dim3(std::max(1, grid / 4)), dim3(1024), 0, s, P);
         if (sph) hipLaunchKernelGGL(k_vol_flight<true>, dim3(grid), dim3(256), 0, s, S, P);
         else hipLaunchKernelGGL(k_vol_flight<false>, dim3(grid), dim3(256), 0, s, S, P);
         vol_launch_check();
