@@ -655,19 +655,29 @@ __global__ __launch_bounds__(256) void k_shadow(DeviceScene S, PathBufs P) {
     uint32_t nn = 0, np = 0;
     Tracer<true, STATS, SPH> tr;
     bool busy = false, fin = false;
-    int i = 0, next = gtid;
+    // each wave walks its own contiguous share of the queue (neighbouring
+    // rays come from neighbouring paths), its idle lanes taking the next
+    // rays of it in order
+    const int nwaves = stride >> 6;
+    const int per = (n + nwaves - 1) / nwaves;
+    int cursor = (gtid >> 6) * per;
+    const int end = min(n, cursor + per);
+    int i = 0;
     while (true) {
-        const int nidle = __popcll(__ballot(!busy));
+        const uint64_t idle = __ballot(!busy);
+        const int nidle = __popcll(idle);
         if (nidle >= kShadowRefill) {
             if (fin) P.sh_occ[i] = tr.found ? 1 : 0;
             fin = false;
-            if (!busy && next < n) {
-                i = next;
-                next += stride;
+            const int k = cursor + int(__builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0)));
+            if (!busy && k < end) {
+                i = k;
                 const float4 o = P.sh_o[i], d = P.sh_d[i];
                 tr.start(o.x, o.y, o.z, d.x, d.y, d.z, o.w, d.w);
                 busy = true;
             }
+            cursor = min(end, cursor + nidle);
             if (__ballot(busy) == 0) break;
         }
         if (busy) {

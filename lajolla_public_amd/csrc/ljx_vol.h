@@ -472,15 +472,23 @@ __global__ __launch_bounds__(256) void k_vol_occlude(DeviceScene S, PathBufs P) 
     unsigned long long segs = 0;
     Tracer<true, STATS, SPH, true> tr;
     bool busy = false, fin = false;
-    int i = 0, next = gtid;
+    // each wave walks its own contiguous share of the queue (k_shadow)
+    const int nwaves = stride >> 6;
+    const int per = (n + nwaves - 1) / nwaves;
+    int cursor = (gtid >> 6) * per;
+    const int end = min(n, cursor + per);
+    int i = 0;
     while (true) {
-        const int nidle = __popcll(__ballot(!busy));
+        const uint64_t idle = __ballot(!busy);
+        const int nidle = __popcll(idle);
         if (nidle >= kShadowRefill) {
             if (fin) P.sh_occ[i] = tr.found ? 1 : (tr.passed ? 2 : 0);
             fin = false;
-            if (!busy && next < n) {
-                i = next;
-                next += stride;
+            const int k = cursor + int(__builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0)));
+            cursor = min(end, cursor + nidle);
+            if (!busy && k < end) {
+                i = k;
                 const float4 so = P.sh_o[i], sd = P.sh_d[i];
                 const D3 p{so.x, so.y, so.z}, pp{sd.x, sd.y, sd.z};
                 const D3 dir_light = normalize(pp - p);
