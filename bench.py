@@ -353,7 +353,7 @@ def rel_mse(img, ref, trim=0.0):
 TRIM = 1e-4  # the 0.01% largest per-pixel errors (207 pixels at 1080p)
 
 
-def guided_vs_plain(ref_spp, plain_spp=64, train_spp=8, steps_per_pass=6, plain_seeds=8):
+def guided_vs_plain(ref_spp, plain_spp=64, train_spp=4, steps_per_pass=3, plain_seeds=8):
     """relMSE at equal wall time, learned guiding vs the plain tracer, on the
     guiding workload (terrain_gap: the large-mesh scene under a ceiling with a
     5% gap, 90% of the light power behind it), 1920x1080, depth 8.  Each
