@@ -226,12 +226,22 @@ struct Scene {
     bool tuned = false;
     ljx_exec_options tuned_exec{};  // autotuned launch settings
 
+    // side streams the shading kernels of one bounce fan out to (default
+    // priority, like the scene's stream), with the events that fork and join them
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+
     ~Scene() {
         if (device >= 0) hipSetDevice(device);
         if (stream) hipStreamSynchronize(stream);
         release_ctx();
         for (auto &b : bufs)
             if (b.p) hipFree(b.p);
+        for (int k = 0; k < 2; k++) {
+            if (side[k]) hipStreamDestroy(side[k]);
+            if (ev_join[k]) hipEventDestroy(ev_join[k]);
+        }
+        if (ev_fork) hipEventDestroy(ev_fork);
         if (stream) hipStreamDestroy(stream);
     }
     void release_ctx() {
@@ -1032,30 +1042,58 @@ void launch(K kernel, int grid, int block, hipStream_t s, A... args) {
     HIPX(hipGetLastError());
 }
 
+// The shading kernels of one bounce work on disjoint buckets (their only
+// shared writes are atomic appends to the shadow queue), so with fan set
+// they run on three streams at once: a long kernel at one wave per SIMD
+// (DisneyBSDF) leaves most of each SIMD to the others, and the launches'
+// tails overlap.  Captured work (batches) keeps them on one stream.
 template <bool SPH, bool GUIDE>
-void shade_all_t(Scene &sc, PathBufs &P, int cur, int grid, int block, hipStream_t s) {
+void shade_all_t(Scene &sc, PathBufs &P, int cur, int grid, int block, hipStream_t s, bool fan) {
     const DeviceScene &S = sc.ds;
     unsigned m = sc.mat_types_present;
-    if (m & (1u << 0)) launch(k_shade_w2<0, SPH, GUIDE>, grid, block, s, S, P, cur);
-    if (m & (1u << 1)) launch(k_shade_w2<1, SPH, GUIDE>, grid, block, s, S, P, cur);
-    if (m & (1u << 2)) launch(k_shade<2, SPH, GUIDE>, grid, block, s, S, P, cur);
-    if (m & (1u << 3)) launch(k_shade_w2<3, SPH, GUIDE>, grid, block, s, S, P, cur);
-    if (m & (1u << 4)) launch(k_shade_w2<4, SPH, GUIDE>, grid, block, s, S, P, cur);
-    if (m & (1u << 5)) launch(k_shade<5, SPH, GUIDE>, grid, block, s, S, P, cur);
-    if (m & (1u << 6)) launch(k_shade<6, SPH, GUIDE>, grid, block, s, S, P, cur);
-    if (m & (1u << 7)) launch(k_shade_w2<7, SPH, GUIDE>, grid, block, s, S, P, cur);
-    if (m & (1u << 8)) launch(k_shade<8, SPH, GUIDE>, grid, block, s, S, P, cur);
+    hipStream_t st[3] = {s, s, s};
+    if (fan) {
+        for (int k = 0; k < 2; k++) {
+            if (!sc.side[k]) HIPX(hipStreamCreateWithFlags(&sc.side[k], hipStreamNonBlocking));
+            if (!sc.ev_join[k]) HIPX(hipEventCreateWithFlags(&sc.ev_join[k], hipEventDisableTiming));
+        }
+        if (!sc.ev_fork) HIPX(hipEventCreateWithFlags(&sc.ev_fork, hipEventDisableTiming));
+        HIPX(hipEventRecord(sc.ev_fork, s));
+        for (int k = 0; k < 2; k++) {
+            HIPX(hipStreamWaitEvent(sc.side[k], sc.ev_fork, 0));
+            st[k + 1] = sc.side[k];
+        }
+    }
+    int next = 0;
+    auto on = [&]() { return st[next++ % 3]; };
+    // the heaviest kernels first, one per stream
+    if (m & (1u << 1)) launch(k_shade_w2<1, SPH, GUIDE>, grid, block, on(), S, P, cur);
+    if (m & (1u << 8)) launch(k_shade<8, SPH, GUIDE>, grid, block, on(), S, P, cur);
+    if (m & (1u << 0)) launch(k_shade_w2<0, SPH, GUIDE>, grid, block, on(), S, P, cur);
+    if (m & (1u << 2)) launch(k_shade<2, SPH, GUIDE>, grid, block, on(), S, P, cur);
+    if (m & (1u << 3)) launch(k_shade_w2<3, SPH, GUIDE>, grid, block, on(), S, P, cur);
+    if (m & (1u << 4)) launch(k_shade_w2<4, SPH, GUIDE>, grid, block, on(), S, P, cur);
+    if (m & (1u << 5)) launch(k_shade<5, SPH, GUIDE>, grid, block, on(), S, P, cur);
+    if (m & (1u << 6)) launch(k_shade<6, SPH, GUIDE>, grid, block, on(), S, P, cur);
+    if (m & (1u << 7)) launch(k_shade_w2<7, SPH, GUIDE>, grid, block, on(), S, P, cur);
     // unsorted bounces and textured materials go to the runtime-dispatch kernel
-    if (P.sort_threshold > 0 || sc.has_textured) launch(k_shade<-1, SPH, GUIDE>, grid, block, s, S, P, cur);
-    launch(k_miss, grid, block, s, S, P);
+    if (P.sort_threshold > 0 || sc.has_textured) launch(k_shade<-1, SPH, GUIDE>, grid, block, on(), S, P, cur);
+    launch(k_miss, grid, block, on(), S, P);
+    if (fan) {
+        for (int k = 0; k < 2; k++) {
+            HIPX(hipEventRecord(sc.ev_join[k], sc.side[k]));
+            HIPX(hipStreamWaitEvent(s, sc.ev_join[k], 0));
+        }
+    }
 }
-void shade_all(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s, bool guide, int block = 256) {
+void shade_all(Scene &sc, PathBufs &P, int cur, int grid, hipStream_t s, bool guide, int block = 256,
+               bool fan = false) {
     if (guide) {
-        if (sc.has_spheres) shade_all_t<true, true>(sc, P, cur, grid, block, s);
-        else shade_all_t<false, true>(sc, P, cur, grid, block, s);
+        if (sc.has_spheres) shade_all_t<true, true>(sc, P, cur, grid, block, s, fan);
+        else shade_all_t<false, true>(sc, P, cur, grid, block, s, fan);
     } else {
-        if (sc.has_spheres) shade_all_t<true, false>(sc, P, cur, grid, block, s);
-        else shade_all_t<false, false>(sc, P, cur, grid, block, s);
+        if (sc.has_spheres) shade_all_t<true, false>(sc, P, cur, grid, block, s, fan);
+        else shade_all_t<false, false>(sc, P, cur, grid, block, s, fan);
     }
 }
 template <bool STATS, bool SPH>
@@ -1306,7 +1344,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
                 }
                 tm.begin(2, ev);
                 if (vol) vol_shade_all(S, P, cur, shade_grid, s, sc.has_spheres, (sc.mat_types_present & ~7u) == 0);
-                else shade_all(sc, P, cur, shade_grid, s, guided, shade_block);
+                else shade_all(sc, P, cur, shade_grid, s, guided, shade_block, /*fan=*/true);
                 build_ext_queue(P, ns * nq, cur, s);
                 tm.end(2, ev);
                 tm.begin(1, ev);
