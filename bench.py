@@ -38,6 +38,8 @@ def main():
     ap.add_argument("--max-paths", type=int, default=0)
     ap.add_argument("--no-guide", action="store_true", help="skip the guided-vs-plain equal-time comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the 96-job batched-rendering run")
+    ap.add_argument("--no-vol", action="store_true",
+                    help="skip the volumetric workloads (nested media for each phase g, the smoke grid)")
     ap.add_argument("--guide-ref-spp", type=int, default=2048,
                     help="spp of the plain reference image the relMSE figures are measured against")
     ap.add_argument("--mode", choices=["render", "guided", "copycal"], default="render",
@@ -152,11 +154,13 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(desc, args.cpu_spp)
-    guiding = batch = None
+    guiding = batch = volumetric = None
     if args.workload == "terrain_spheres":
         if not args.no_batch:
             batch = batched_jobs(sc, desc)
         del sc
+        if not args.no_vol:
+            volumetric = volumetric_workloads()
         if not args.no_guide:
             guiding = guided_vs_plain(args.guide_ref_spp)
 
@@ -227,6 +231,7 @@ def main():
         "cpu_baseline": cpu,
         "guiding": guiding,
         "batch": batch,
+        "volumetric": volumetric,
     }
     if cpu and cpu.get("value"):
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
@@ -296,6 +301,37 @@ def guided_mode(args):
                          1) if t_ms > 0 else None},
     }
     print(json.dumps(line), flush=True)
+
+
+def volumetric_workloads(steps=2):
+    """The volumetric workloads beside the headline line: nested_media at its
+    full size (1024x1024, 128 spp, depth 16, the 100,352-triangle medium
+    sphere) for each Henyey-Greenstein g of the config, and hetvol_smoke
+    (768x576, 256 spp, unbounded depth, 128^3 grid).  One warm-up render,
+    then the best of `steps` renders (wall time of the render call)."""
+    from lajolla_public_amd import api, scenes
+    out = {}
+    jobs = [(f"nested_media_g{g:+.1f}", scenes.nested_media(seed=11, g=g)) for g in (-0.3, 0.0, 0.7)]
+    jobs.append(("hetvol_smoke", scenes.hetvol_smoke(seed=3, spp=256, max_depth=-1, res=128, scale=40.0)))
+    for name, desc in jobs:
+        sc = api.Scene(desc)
+        ex = api.exec_options(collect_timing=1)
+        sc.render(exec_options=ex)
+        best = None
+        for _ in range(steps):
+            t = time.perf_counter()
+            _, st = sc.render(exec_options=ex, stats=True)
+            ms = (time.perf_counter() - t) * 1000
+            if best is None or ms < best[0]:
+                best = (ms, st)
+        ms, st = best
+        W, H, spp = desc.camera["width"], desc.camera["height"], desc.options["samples_per_pixel"]
+        out[name] = {"msamples_per_s": round(W * H * spp / ms / 1e3, 2), "ms": round(ms, 1),
+                     "width": W, "height": H, "spp": spp, "max_depth": desc.options["max_depth"],
+                     "kernel_ms": {k: round(st[k], 1) for k in ("extend_ms", "shadow_ms", "shade_ms", "other_ms")}}
+        print(f"[bench] {name}: {out[name]['msamples_per_s']} Msamples/s ({ms:.0f} ms)", file=sys.stderr, flush=True)
+        sc.close()
+    return out
 
 
 def batched_jobs(sc_c1, desc_c1, n_jobs=96):
