@@ -1068,7 +1068,7 @@ void shade_all_t(Scene &sc, PathBufs &P, int cur, int grid, int block, hipStream
     auto on = [&]() { return st[next++ % 3]; };
     // the heaviest kernels first, one per stream
     if (m & (1u << 1)) launch(k_shade_w2<1, SPH, GUIDE>, grid, block, on(), S, P, cur);
-    if (m & (1u << 8)) launch(k_shade<8, SPH, GUIDE>, grid, block, on(), S, P, cur);
+    if (m & (1u << 8)) launch(k_shade_w2<8, SPH, GUIDE>, grid, block, on(), S, P, cur);
     if (m & (1u << 0)) launch(k_shade_w2<0, SPH, GUIDE>, grid, block, on(), S, P, cur);
     if (m & (1u << 2)) launch(k_shade<2, SPH, GUIDE>, grid, block, on(), S, P, cur);
     if (m & (1u << 3)) launch(k_shade_w2<3, SPH, GUIDE>, grid, block, on(), S, P, cur);
