@@ -233,9 +233,89 @@ struct alignas(16) BvhNode {
     int4 ch;
 };
 
+// The traversal's copy of a 4-wide node (80 bytes, five 16-byte loads instead
+// of seven): the child boxes as IEEE half offsets from the children's common
+// lower corner, in units of a power-of-two scale,
+//   plane = org.xyz + half * org.w,
+// rounded outwards (lower planes down, upper planes up) with a margin of
+// 2^-18 of the node's extent, so every quantised box holds its FP32 box and
+// the arithmetic of the quantised slab test stays on the conservative side.
+// Per axis, qa = (lo0 | lo1 << 16, lo2 | lo3 << 16, hi0 | hi1 << 16,
+// hi2 | hi3 << 16); ch as in Bvh4Node.  A half operand of an FMA is one
+// v_fma_mix_f32 on gfx950, so the slab test costs what the FP32 one does.
+struct alignas(16) Bvh4QNode {
+    float4 org;
+    uint4 qx, qy, qz;
+    int4 ch;
+};
+static_assert(sizeof(Bvh4QNode) == 80, "Bvh4QNode layout");
+
+// IEEE binary16 bits of the largest half <= v (DOWN) or the smallest half >= v.
+template <bool DOWN>
+__host__ __device__ inline uint32_t half_bits_rounded(double v) {
+    if (v < 0) return half_bits_rounded<!DOWN>(-v) ^ 0x8000u;
+    if (v == 0) return 0u;
+    if (v > 65504.0) return DOWN ? 0x7bffu : 0x7c00u;
+    int k;
+    (void)frexp(v, &k);  // v = f * 2^k, f in [0.5, 1)
+    const int e = k - 1;  // 2^e <= v < 2^(e+1)
+    if (e < -14) {        // subnormal: units of 2^-24, bits = units
+        const double u = ldexp(v, 24);
+        return uint32_t(DOWN ? floor(u) : ceil(u));
+    }
+    const double q = ldexp(v, 10 - e);  // in [1024, 2048)
+    const uint32_t m = uint32_t(DOWN ? floor(q) : ceil(q));
+    return (uint32_t(e + 15) << 10) + m - 1024u;  // m == 2048 carries into the exponent
+}
+
+__host__ __device__ inline Bvh4QNode quantize_bvh4(const Bvh4Node &n) {
+    const float lo[3][4] = {{n.lox.x, n.lox.y, n.lox.z, n.lox.w}, {n.loy.x, n.loy.y, n.loy.z, n.loy.w},
+                            {n.loz.x, n.loz.y, n.loz.z, n.loz.w}};
+    const float hi[3][4] = {{n.hix.x, n.hix.y, n.hix.z, n.hix.w}, {n.hiy.x, n.hiy.y, n.hiy.z, n.hiy.w},
+                            {n.hiz.x, n.hiz.y, n.hiz.z, n.hiz.w}};
+    const int ch[4] = {n.ch.x, n.ch.y, n.ch.z, n.ch.w};
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+    bool any = false;
+    for (int c = 0; c < 4; c++) {
+        if (ch[c] == kNoChild) continue;
+        for (int a = 0; a < 3; a++) {
+            bmin[a] = any ? fminf(bmin[a], lo[a][c]) : lo[a][c];
+            bmax[a] = any ? fmaxf(bmax[a], hi[a][c]) : hi[a][c];
+        }
+        any = true;
+    }
+    double ext = 0;
+    for (int a = 0; a < 3; a++) ext = fmax(ext, double(bmax[a]) - double(bmin[a]));
+    int e = 0;
+    if (ext > 0) {
+        int k;
+        (void)frexp(ext, &k);
+        e = k - 14;  // ext / 2^e in [2^13, 2^14)
+        e = e < -60 ? -60 : (e > 60 ? 60 : e);
+    }
+    const double s = ldexp(1.0, e), margin = ldexp(ext, -18);
+    uint32_t q[3][8] = {};
+    for (int c = 0; c < 4; c++) {
+        if (ch[c] == kNoChild) continue;
+        for (int a = 0; a < 3; a++) {
+            q[a][c] = half_bits_rounded<true>((double(lo[a][c]) - double(bmin[a]) - margin) / s);
+            q[a][4 + c] = half_bits_rounded<false>((double(hi[a][c]) - double(bmin[a]) + margin) / s);
+        }
+    }
+    Bvh4QNode o;
+    o.org = make_float4(bmin[0], bmin[1], bmin[2], float(s));
+    uint4 *qa[3] = {&o.qx, &o.qy, &o.qz};
+    for (int a = 0; a < 3; a++)
+        *qa[a] = make_uint4(q[a][0] | q[a][1] << 16, q[a][2] | q[a][3] << 16, q[a][4] | q[a][5] << 16,
+                            q[a][6] | q[a][7] << 16);
+    o.ch = n.ch;
+    return o;
+}
+
 struct DeviceScene {
     // traversal
     const Bvh4Node *nodes;
+    const Bvh4QNode *qnodes;  // the traversal's copy of nodes (quantize_bvh4)
     const float4 *prims;      // 3 float4 per leaf slot: v0|gid, v1|kind, v2|-
     const double4 *spheres;   // centre, radius per sphere primitive
     int num_nodes;

@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int c
         atomicAdd(&P.stat[S_EXT_RAYS], (unsigned long long)n);
     }
     int *fetch = &P.ctr[C_XFETCH + cur];
-    const TraceScene ts{S.nodes, S.prims, S.spheres};
+    const TraceScene ts{S.qnodes, S.prims, S.spheres};
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int sstride = gridDim.x * blockDim.x;
     int *spill = P.spill + gtid;
@@ -651,7 +651,7 @@ __global__ __launch_bounds__(256) void k_shadow(DeviceScene S, PathBufs P) {
         atomicAdd(&P.stat[S_SH_RAYS], (unsigned long long)n);
         for (int b = 0; b < kNumBuckets; b++) P.ctr[C_BUCKET + b] = 0;
     }
-    const TraceScene ts{S.nodes, S.prims, S.spheres};
+    const TraceScene ts{S.qnodes, S.prims, S.spheres};
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
     int *spill = P.spill + gtid;
@@ -788,7 +788,7 @@ __global__ __launch_bounds__(256) void k_aux(DeviceScene S, PathBufs P, int mode
         sample_primary(S.camera, (x + 0.5) / P.W, (y + 0.5) / P.H, o, d);
         TraceHit h;
         bool hit = S.num_nodes > 0 &&
-                   trace_ray<false, false, SPH>(TraceScene{S.nodes, S.prims, S.spheres}, float(o.x), float(o.y),
+                   trace_ray<false, false, SPH>(TraceScene{S.qnodes, S.prims, S.spheres}, float(o.x), float(o.y),
                                                 float(o.z), float(d.x), float(d.y), float(d.z), 0.f,
                                                 __builtin_huge_valf(), stack + threadIdx.x, blockDim.x, spill, stride,
                                                 h, nn, np);

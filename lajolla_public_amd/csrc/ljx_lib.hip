@@ -402,6 +402,12 @@ std::vector<PrimBox> host_prim_boxes(const std::vector<float> &vpos, const std::
     return boxes;
 }
 
+__global__ __launch_bounds__(256) void k_quantize_nodes(const Bvh4Node *__restrict__ in, int n,
+                                                        Bvh4QNode *__restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = quantize_bvh4(in[i]);
+}
+
 // Builds the 4-wide BVH over the scene's primitives (on the device, or on the
 // host when the scene asks for it; both give the same tree), then the leaf-slot
 // records and the shading bucket of every slot.  The geometry arrays are
@@ -415,6 +421,7 @@ void build_bvh(Scene &sc, const ljx_scene_desc *d, const std::vector<MatRec> &ma
     sc.num_slots = nprim;
     if (nprim == 0) {
         S.nodes = sc.upload(std::vector<Bvh4Node>());
+        S.qnodes = sc.upload(std::vector<Bvh4QNode>());
         S.num_nodes = 0;
         S.prims = sc.upload(std::vector<float4>());
         S.num_slots = 0;
@@ -457,6 +464,13 @@ void build_bvh(Scene &sc, const ljx_scene_desc *d, const std::vector<MatRec> &ma
         sc.bvh_depth = bvh4.depth;
         sc.binary_depth = bvh.max_depth;
         stack_need = bvh4.stack_need;
+    }
+    {
+        Bvh4QNode *q = static_cast<Bvh4QNode *>(keep(size_t(S.num_nodes) * sizeof(Bvh4QNode)));
+        const int n = S.num_nodes;
+        hipLaunchKernelGGL(k_quantize_nodes, dim3((n + 255) / 256), dim3(256), 0, sc.stream, S.nodes, n, q);
+        HIPX(hipGetLastError());
+        S.qnodes = q;
     }
     sc.stack_need = stack_need;
     sc.spill_entries = std::max(1, stack_need - kStackLds);
@@ -831,11 +845,11 @@ __global__ __launch_bounds__(256) void k_trace_batch(DeviceScene S, const ljx_ra
         bool ok = false;
         if (S.num_nodes > 0) {
             if (stats)
-                ok = trace_ray<ANY, true, SPH>(TraceScene{S.nodes, S.prims, S.spheres}, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1],
+                ok = trace_ray<ANY, true, SPH>(TraceScene{S.qnodes, S.prims, S.spheres}, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1],
                                           r.dir[2], r.tnear, r.tfar, stack + threadIdx.x, blockDim.x,
                                           spill + gtid, stride, h, nn, np);
             else
-                ok = trace_ray<ANY, false, SPH>(TraceScene{S.nodes, S.prims, S.spheres}, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1],
+                ok = trace_ray<ANY, false, SPH>(TraceScene{S.qnodes, S.prims, S.spheres}, r.org[0], r.org[1], r.org[2], r.dir[0], r.dir[1],
                                            r.dir[2], r.tnear, r.tfar, stack + threadIdx.x, blockDim.x,
                                            spill + gtid, stride, h, nn, np);
         }
@@ -1183,7 +1197,7 @@ void aux_render(Scene &sc, const ljx_render_options &opt, const ljx_exec_options
         stats->extension_rays = stats->camera_paths;
         stats->build_ms = sc.build_ms;
         stats->bvh_depth = sc.bvh_depth;
-        stats->node_bytes = 7 * 16;
+        stats->node_bytes = sizeof(Bvh4QNode);
         stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     }
 }
@@ -1413,7 +1427,7 @@ void render_impl(Scene &sc, const ljx_render_options *o, const ljx_exec_options 
         stats->instrumented_rays = stats_on ? st[S_EXT_RAYS] + st[S_SH_RAYS] : 0;
         stats->build_ms = sc.build_ms;
         stats->bvh_depth = sc.bvh_depth;
-        stats->node_bytes = 7 * 16;  // six SoA box vectors and the child references
+        stats->node_bytes = sizeof(Bvh4QNode);  // corner and scale, three half-box vectors, child references
         stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     } else {
         for (int k = 0; k < 6; k++) tm.total(k);
@@ -1759,7 +1773,7 @@ void trace_device(Scene &sc, const ljx_ray *rays, size_t n, void *out, const ljx
         }
         stats->build_ms = sc.build_ms;
         stats->bvh_depth = sc.bvh_depth;
-        stats->node_bytes = 7 * 16;  // six SoA box vectors and the child references
+        stats->node_bytes = sizeof(Bvh4QNode);  // corner and scale, three half-box vectors, child references
         stats->instrumented_rays = st_on ? n : 0;
     }
 }
@@ -1886,7 +1900,7 @@ int ljx_scene_bvh_info(void *scene, ljx_bvh_info *info) {
 }
 
 // Random node fetches over the scene's 4-wide BVH: each of `lanes` lanes
-// of every wave reads whole nodes (the traversal's seven 16-byte loads) at
+// of every wave reads whole nodes (the traversal's five 16-byte loads) at
 // hashed indices, the next index depending on the data just loaded, on the
 // traversal's grid.  This is the traversal's memory access pattern with no
 // coherence between the lanes of a wave and no arithmetic in the way (a
@@ -1900,17 +1914,18 @@ __device__ __forceinline__ uint32_t fetch_hash(uint32_t x) {
     x ^= x >> 16;
     return x;
 }
-__global__ __launch_bounds__(256) void k_node_fetch(const Bvh4Node *__restrict__ nodes, uint32_t n, int iters,
+__global__ __launch_bounds__(256) void k_node_fetch(const Bvh4QNode *__restrict__ nodes, uint32_t n, int iters,
                                                     int lanes, float *sink) {
     uint32_t h = fetch_hash(blockIdx.x * 256u + threadIdx.x);
     float acc = 0.f;
     if (int(threadIdx.x & 63) < lanes) {
         for (int it = 0; it < iters; it++) {
             h = fetch_hash(h + uint32_t(it));
-            const Bvh4Node *nd = nodes + h % n;
-            const float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
+            const Bvh4QNode *nd = nodes + h % n;
+            const float4 org = nd->org;
+            const uint4 qx = nd->qx, qy = nd->qy, qz = nd->qz;
             const int4 ch = nd->ch;
-            acc += lx.x + hx.y + ly.z + hy.w + lz.x + hz.y + float(ch.z & 1);
+            acc += org.x + float(qx.y & 1) + float(qy.z & 1) + float(qz.w & 1) + float(ch.z & 1);
             h ^= __float_as_uint(acc) & 1u;
         }
     }
@@ -1933,7 +1948,7 @@ int ljx_scene_node_fetch_rate(void *scene, int lanes, double *fetches_per_s) {
         float best = 0;
         for (int rep = 0; rep < 5; rep++) {
             HIPX(hipEventRecord(e0, 0));
-            hipLaunchKernelGGL(k_node_fetch, dim3(grid), dim3(256), 0, 0, sc->ds.nodes, uint32_t(sc->ds.num_nodes),
+            hipLaunchKernelGGL(k_node_fetch, dim3(grid), dim3(256), 0, 0, sc->ds.qnodes, uint32_t(sc->ds.num_nodes),
                                iters, lanes, sink);
             HIPX(hipGetLastError());
             HIPX(hipEventRecord(e1, 0));

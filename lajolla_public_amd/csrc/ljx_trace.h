@@ -2,8 +2,9 @@
 //
 // Replaces Embree's rtcIntersect1 / rtcOccluded1 as lajolla calls them
 // (src/intersection.cpp:7-85).  Node layout: a 4-wide BVH, the four child
-// boxes of a node in SoA form in 128 bytes (ljx_device.h Bvh4Node), loaded as
-// seven 16-byte vector loads; hit children are ordered near to far with a
+// boxes of a node as outward-rounded half offsets from a common corner, in
+// 80 bytes (ljx_device.h Bvh4QNode, quantised from the builders' FP32
+// Bvh4Node), loaded as five 16-byte vector loads; hit children are ordered near to far with a
 // five-comparator sorting network.  The traversal stack lives in LDS
 // (kStackLds entries per lane, lane-interleaved so a wave's pushes hit 64
 // consecutive banks) and spills to a per-lane global area beyond that, sized
@@ -72,16 +73,31 @@ __device__ __forceinline__ RayPre ray_setup(float ox, float oy, float oz, float 
     return r;
 }
 
-// Slab test of one child box; returns entry distance, or +inf on a miss.
-__device__ __forceinline__ float box_entry(const RayPre &r, float lx, float hx, float ly, float hy, float lz, float hz,
-                                           float tmin, float tmax) {
-    float tx0 = __builtin_fmaf(lx, r.idx, -r.oidx), tx1 = __builtin_fmaf(hx, r.idx, -r.oidx);
-    float ty0 = __builtin_fmaf(ly, r.idy, -r.oidy), ty1 = __builtin_fmaf(hy, r.idy, -r.oidy);
-    float tz0 = __builtin_fmaf(lz, r.idz, -r.oidz), tz1 = __builtin_fmaf(hz, r.idz, -r.oidz);
+// Half value of bits [16*HI, 16*HI+16) of w, widened to FP32 (exactly).  As
+// an FMA operand the widening folds into v_fma_mix_f32.
+template <int HI>
+__device__ __forceinline__ float half_at(uint32_t w) {
+    return float(__builtin_bit_cast(_Float16, uint16_t(HI ? (w >> 16) : (w & 0xffffu))));
+}
+
+// Slab test of child C of a quantised node (Bvh4QNode).  s*: 1/d scaled by
+// the node's scale; f*: the distance to the node's lower corner plane
+// (org * 1/d - o * 1/d).  Returns the entry distance, or +inf on a miss.
+// The far distance is widened by 1 + 2 * gamma(4) in FP32 (Ize 2013 widens by
+// 1 + 2 * gamma(3)); the extra rounding of f* is covered by that and by the
+// quantiser's margin.
+template <int C>
+__device__ __forceinline__ float qbox_entry(float sx, float fx, float sy, float fy, float sz, float fz, uint4 qx,
+                                            uint4 qy, uint4 qz, float tmin, float tmax) {
+    constexpr int W = C >> 1, H = C & 1;
+    auto lo = [](uint4 q) { return half_at<H>(W ? q.y : q.x); };
+    auto hi = [](uint4 q) { return half_at<H>(W ? q.w : q.z); };
+    float tx0 = __builtin_fmaf(lo(qx), sx, fx), tx1 = __builtin_fmaf(hi(qx), sx, fx);
+    float ty0 = __builtin_fmaf(lo(qy), sy, fy), ty1 = __builtin_fmaf(hi(qy), sy, fy);
+    float tz0 = __builtin_fmaf(lo(qz), sz, fz), tz1 = __builtin_fmaf(hi(qz), sz, fz);
     float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
     float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-    // conservative widening (Ize 2013): 1 + 2 * gamma(3) in FP32
-    tf *= 1.00000036f;
+    tf *= 1.00000048f;
     return tn <= tf ? tn : __builtin_huge_valf();
 }
 
@@ -170,7 +186,7 @@ __device__ __forceinline__ bool sphere_roots(double ox, double oy, double oz, do
 }
 
 struct TraceScene {
-    const Bvh4Node *__restrict__ nodes;
+    const Bvh4QNode *__restrict__ nodes;
     const float4 *__restrict__ prims;    // 3 per leaf slot
     const double4 *__restrict__ spheres; // centre xyz, radius
     const int8_t *__restrict__ skip = nullptr;  // OPAQUE traversals: per slot, nonzero to pass through
@@ -221,14 +237,18 @@ struct Tracer {
             }
         };
         if (node >= 0) {
-            const Bvh4Node *nd = ts.nodes + node;
-            float4 lx = nd->lox, hx = nd->hix, ly = nd->loy, hy = nd->hiy, lz = nd->loz, hz = nd->hiz;
+            const Bvh4QNode *nd = ts.nodes + node;
+            const float4 org = nd->org;
+            const uint4 qx = nd->qx, qy = nd->qy, qz = nd->qz;
             int4 ch = nd->ch;
             if (STATS) n_nodes++;
-            float t0 = box_entry(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tmin, tmax);
-            float t1 = box_entry(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tmin, tmax);
-            float t2 = box_entry(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tmin, tmax);
-            float t3 = box_entry(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tmin, tmax);
+            const float sx = r.idx * org.w, sy = r.idy * org.w, sz = r.idz * org.w;
+            const float fx = __builtin_fmaf(org.x, r.idx, -r.oidx), fy = __builtin_fmaf(org.y, r.idy, -r.oidy),
+                        fz = __builtin_fmaf(org.z, r.idz, -r.oidz);
+            float t0 = qbox_entry<0>(sx, fx, sy, fy, sz, fz, qx, qy, qz, tmin, tmax);
+            float t1 = qbox_entry<1>(sx, fx, sy, fy, sz, fz, qx, qy, qz, tmin, tmax);
+            float t2 = qbox_entry<2>(sx, fx, sy, fy, sz, fz, qx, qy, qz, tmin, tmax);
+            float t3 = qbox_entry<3>(sx, fx, sy, fy, sz, fz, qx, qy, qz, tmin, tmax);
             int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
             if (c2 == kNoChild) t2 = kMiss;  // empty slots carry inverted boxes, which
             if (c3 == kNoChild) t3 = kMiss;  // a slab test alone does not reject

@@ -464,7 +464,7 @@ template <bool STATS, bool SPH>
 __global__ __launch_bounds__(256) void k_vol_occlude(DeviceScene S, PathBufs P) {
     __shared__ int stack[kStackLds * 256];
     const int n = P.ctr[C_SH];
-    const TraceScene ts{S.nodes, S.prims, S.spheres, S.slot_passthru};
+    const TraceScene ts{S.qnodes, S.prims, S.spheres, S.slot_passthru};
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int stride = gridDim.x * blockDim.x;
     int *spill = P.spill + gtid;
@@ -556,7 +556,7 @@ __global__ __launch_bounds__(256) void k_vol_shadow(DeviceScene S, PathBufs P) {
             TraceHit h;
             if (crossings) segs++;
             bool hit = crossings &&
-                       trace_ray<false, STATS, SPH>(TraceScene{S.nodes, S.prims, S.spheres}, float(p0.x), float(p0.y),
+                       trace_ray<false, STATS, SPH>(TraceScene{S.qnodes, S.prims, S.spheres}, float(p0.x), float(p0.y),
                                                     float(p0.z), float(dir_light.x), float(dir_light.y),
                                                     float(dir_light.z), float(S.eps), float((1 - S.eps) * dl),
                                                     stack + threadIdx.x, blockDim.x, spill, stride, h, nn, np);
