@@ -406,9 +406,14 @@ typedef struct ljx_bvh_info {
     double sah_cost;               /* expected node visits + primitive tests of a ray through the root box */
     double bvh_ms;                 /* wall time of the BVH build (boxes, tree, slot records) */
     double build_ms;               /* wall time of the whole scene build */
+    uint64_t qnode_hash;           /* FNV-1a (64-bit) of the traversal's quantised nodes (80 B each) */
+    uint64_t qnode_bad_planes;     /* quantised box planes that do not hold their FP32 plane (0 expected) */
+    double qnode_max_loose;        /* largest outward move of a quantised plane, over its node's extent */
 } ljx_bvh_info;
 
-/* Fills info; downloads the nodes, so it is for tests and tools, not loops. */
+/* Fills info; downloads the nodes (both the built FP32 nodes and the
+ * traversal's quantised copy, which it checks against them), so it is for
+ * tests and tools, not loops. */
 int ljx_scene_bvh_info(void *scene, ljx_bvh_info *info);
 
 /* Measurement helper with no counterpart in the reference: the rate

@@ -318,6 +318,9 @@ class BvhInfo(C.Structure):
         ("sah_cost", C.c_double),
         ("bvh_ms", C.c_double),
         ("build_ms", C.c_double),
+        ("qnode_hash", C.c_uint64),
+        ("qnode_bad_planes", C.c_uint64),
+        ("qnode_max_loose", C.c_double),
     ]
 
 

@@ -268,6 +268,13 @@ __host__ __device__ inline uint32_t half_bits_rounded(double v) {
     return (uint32_t(e + 15) << 10) + m - 1024u;  // m == 2048 carries into the exponent
 }
 
+// Value of IEEE binary16 bits b.
+__host__ __device__ inline double half_value(uint32_t b) {
+    const int e = (b >> 10) & 31, m = b & 1023;
+    const double v = e == 0 ? ldexp(double(m), -24) : (e == 31 ? (m ? NAN : INFINITY) : ldexp(double(1024 + m), e - 25));
+    return (b & 0x8000u) ? -v : v;
+}
+
 __host__ __device__ inline Bvh4QNode quantize_bvh4(const Bvh4Node &n) {
     const float lo[3][4] = {{n.lox.x, n.lox.y, n.lox.z, n.lox.w}, {n.loy.x, n.loy.y, n.loy.z, n.loy.w},
                             {n.loz.x, n.loz.y, n.loz.z, n.loz.w}};

@@ -1896,6 +1896,39 @@ int ljx_scene_bvh_info(void *scene, ljx_bvh_info *info) {
                 }
             info->sah_cost = cost;
         }
+        // the traversal's quantised copy: its fingerprint, and every box plane
+        // checked against the FP32 plane it stands for
+        std::vector<Bvh4QNode> qn(sc->ds.num_nodes);
+        if (!qn.empty())
+            HIPX(hipMemcpy(qn.data(), sc->ds.qnodes, qn.size() * sizeof(Bvh4QNode), hipMemcpyDeviceToHost));
+        info->qnode_hash = fnv(qn.data(), qn.size() * sizeof(Bvh4QNode));
+        uint64_t bad = 0;
+        double loose = 0;
+        for (size_t i = 0; i < qn.size(); i++) {
+            const Bvh4Node &nd = nodes[i];
+            const Bvh4QNode &q = qn[i];
+            const float *lo[3] = {&nd.lox.x, &nd.loy.x, &nd.loz.x}, *hi[3] = {&nd.hix.x, &nd.hiy.x, &nd.hiz.x};
+            const uint32_t *qa[3] = {&q.qx.x, &q.qy.x, &q.qz.x};
+            const double org[3] = {q.org.x, q.org.y, q.org.z};
+            double bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int c = 0; c < 4; c++)
+                if ((&nd.ch.x)[c] != kNoChild)
+                    for (int a = 0; a < 3; a++) bmin[a] = std::min<double>(bmin[a], lo[a][c]), bmax[a] = std::max<double>(bmax[a], hi[a][c]);
+            const double ext = std::max({bmax[0] - bmin[0], bmax[1] - bmin[1], bmax[2] - bmin[2]});
+            if ((&q.ch.x)[0] != nd.ch.x || q.ch.y != nd.ch.y || q.ch.z != nd.ch.z || q.ch.w != nd.ch.w) bad++;
+            for (int c = 0; c < 4; c++) {
+                if ((&nd.ch.x)[c] == kNoChild) continue;
+                for (int a = 0; a < 3; a++) {
+                    const double L = org[a] + half_value((qa[a][c >> 1] >> (16 * (c & 1))) & 0xffffu) * q.org.w;
+                    const double H = org[a] + half_value((qa[a][2 + (c >> 1)] >> (16 * (c & 1))) & 0xffffu) * q.org.w;
+                    if (!(L <= lo[a][c])) bad++;
+                    if (!(H >= hi[a][c])) bad++;
+                    if (ext > 0) loose = std::max(loose, std::max(lo[a][c] - L, H - hi[a][c]) / ext);
+                }
+            }
+        }
+        info->qnode_bad_planes = bad;
+        info->qnode_max_loose = loose;
     });
 }
 

@@ -4,7 +4,8 @@ Both builders take their split decisions from the same code
 (csrc/ljx_bvh_sah.h, no FP contraction) and partition stably, so on the same
 primitive boxes the 4-wide nodes (in depth-first order) and the leaf-slot
 table must be identical.  ljx_scene_bvh_info fingerprints both (FNV-1a) and
-reports the SAH cost.  Cases: the Cornell box, spheres with meshes, the
+reports the SAH cost, and checks the traversal's quantised nodes against the
+FP32 nodes they are made from.  Cases: the Cornell box, spheres with meshes, the
 2,308,746-triangle terrain scene, the 100,352-triangle sphere of the nested
 media scene, degenerate input (many triangles with one centroid, which has no
 binned split), a single triangle (the root is a leaf) and a range just above
@@ -38,6 +39,12 @@ def check_same(desc, label):
     for k in ("num_nodes", "num_slots", "depth", "stack_need", "binary_depth", "node_hash", "slot_hash"):
         assert gi[k] == hi[k], (label, k, gi[k], hi[k])
     assert gi["sah_cost"] == hi["sah_cost"]
+    # the traversal's quantised copy (made on the device after either build):
+    # the same bytes, and every half-precision box plane outside or on the
+    # FP32 plane it stands for, at most ~2^-10 of its node's extent away
+    assert gi["qnode_hash"] == hi["qnode_hash"], label
+    assert gi["qnode_bad_planes"] == 0 and hi["qnode_bad_planes"] == 0, (label, gi["qnode_bad_planes"])
+    assert 0 <= gi["qnode_max_loose"] <= 2.0 ** -9, (label, gi["qnode_max_loose"])
     return gi, hi
 
 
