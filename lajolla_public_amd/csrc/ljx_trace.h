@@ -80,6 +80,19 @@ __device__ __forceinline__ float half_at(uint32_t w) {
     return float(__builtin_bit_cast(_Float16, uint16_t(HI ? (w >> 16) : (w & 0xffffu))));
 }
 
+// The half planes of one axis of a quantised node, ordered for the ray:
+// n = the near planes' words (children 0|1, 2|3), f = the far planes'.  Along
+// a positive direction the lower planes are the near ones, along a negative
+// one the upper planes; this is the order min/max of the two slab distances
+// would give, since an FMA with one positive (negative) factor is monotone
+// in the other.
+struct AxisPlanes {
+    uint32_t n0, n1, f0, f1;
+};
+__device__ __forceinline__ AxisPlanes axis_planes(uint4 q, bool neg) {
+    return AxisPlanes{neg ? q.z : q.x, neg ? q.w : q.y, neg ? q.x : q.z, neg ? q.y : q.w};
+}
+
 // Slab test of child C of a quantised node (Bvh4QNode).  s*: 1/d scaled by
 // the node's scale; f*: the distance to the node's lower corner plane
 // (org * 1/d - o * 1/d).  Returns the entry distance, or +inf on a miss.
@@ -87,16 +100,17 @@ __device__ __forceinline__ float half_at(uint32_t w) {
 // 1 + 2 * gamma(3)); the extra rounding of f* is covered by that and by the
 // quantiser's margin.
 template <int C>
-__device__ __forceinline__ float qbox_entry(float sx, float fx, float sy, float fy, float sz, float fz, uint4 qx,
-                                            uint4 qy, uint4 qz, float tmin, float tmax) {
+__device__ __forceinline__ float qbox_entry(float sx, float fx, float sy, float fy, float sz, float fz,
+                                            const AxisPlanes &px, const AxisPlanes &py, const AxisPlanes &pz,
+                                            float tmin, float tmax) {
     constexpr int W = C >> 1, H = C & 1;
-    auto lo = [](uint4 q) { return half_at<H>(W ? q.y : q.x); };
-    auto hi = [](uint4 q) { return half_at<H>(W ? q.w : q.z); };
-    float tx0 = __builtin_fmaf(lo(qx), sx, fx), tx1 = __builtin_fmaf(hi(qx), sx, fx);
-    float ty0 = __builtin_fmaf(lo(qy), sy, fy), ty1 = __builtin_fmaf(hi(qy), sy, fy);
-    float tz0 = __builtin_fmaf(lo(qz), sz, fz), tz1 = __builtin_fmaf(hi(qz), sz, fz);
-    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
-    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+    auto nr = [](const AxisPlanes &p) { return half_at<H>(W ? p.n1 : p.n0); };
+    auto fr = [](const AxisPlanes &p) { return half_at<H>(W ? p.f1 : p.f0); };
+    const float tnx = __builtin_fmaf(nr(px), sx, fx), tfx = __builtin_fmaf(fr(px), sx, fx);
+    const float tny = __builtin_fmaf(nr(py), sy, fy), tfy = __builtin_fmaf(fr(py), sy, fy);
+    const float tnz = __builtin_fmaf(nr(pz), sz, fz), tfz = __builtin_fmaf(fr(pz), sz, fz);
+    const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+    float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
     tf *= 1.00000048f;
     return tn <= tf ? tn : __builtin_huge_valf();
 }
@@ -226,14 +240,27 @@ struct Tracer {
     __device__ __forceinline__ bool step(const TraceScene &ts, int *stack, int stride, int *spill, int spill_stride,
                                          uint32_t &n_nodes, uint32_t &n_prims) {
         const float kMiss = __builtin_huge_valf();
+        // compare-exchange: by selects for closest hits (the branchy form
+        // compiles to exec-masked moves, two more operations each); any hit
+        // keeps the branchy form, which needs fewer registers there
         auto cswap = [](float &ta, int &ca, float &tb, int &cb) {
-            if (tb < ta) {
-                float t = ta;
-                ta = tb;
-                tb = t;
-                int c = ca;
-                ca = cb;
-                cb = c;
+            if (ANY) {
+                if (tb < ta) {
+                    float t = ta;
+                    ta = tb;
+                    tb = t;
+                    int c = ca;
+                    ca = cb;
+                    cb = c;
+                }
+            } else {
+                const bool sw = tb < ta;
+                const float t0 = sw ? tb : ta, t1 = sw ? ta : tb;
+                const int k0 = sw ? cb : ca, k1 = sw ? ca : cb;
+                ta = t0;
+                tb = t1;
+                ca = k0;
+                cb = k1;
             }
         };
         if (node >= 0) {
@@ -245,10 +272,12 @@ struct Tracer {
             const float sx = r.idx * org.w, sy = r.idy * org.w, sz = r.idz * org.w;
             const float fx = __builtin_fmaf(org.x, r.idx, -r.oidx), fy = __builtin_fmaf(org.y, r.idy, -r.oidy),
                         fz = __builtin_fmaf(org.z, r.idz, -r.oidz);
-            float t0 = qbox_entry<0>(sx, fx, sy, fy, sz, fz, qx, qy, qz, tmin, tmax);
-            float t1 = qbox_entry<1>(sx, fx, sy, fy, sz, fz, qx, qy, qz, tmin, tmax);
-            float t2 = qbox_entry<2>(sx, fx, sy, fy, sz, fz, qx, qy, qz, tmin, tmax);
-            float t3 = qbox_entry<3>(sx, fx, sy, fy, sz, fz, qx, qy, qz, tmin, tmax);
+            const AxisPlanes px = axis_planes(qx, r.idx < 0.f), py = axis_planes(qy, r.idy < 0.f),
+                             pz = axis_planes(qz, r.idz < 0.f);
+            float t0 = qbox_entry<0>(sx, fx, sy, fy, sz, fz, px, py, pz, tmin, tmax);
+            float t1 = qbox_entry<1>(sx, fx, sy, fy, sz, fz, px, py, pz, tmin, tmax);
+            float t2 = qbox_entry<2>(sx, fx, sy, fy, sz, fz, px, py, pz, tmin, tmax);
+            float t3 = qbox_entry<3>(sx, fx, sy, fy, sz, fz, px, py, pz, tmin, tmax);
             int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
             if (c2 == kNoChild) t2 = kMiss;  // empty slots carry inverted boxes, which
             if (c3 == kNoChild) t3 = kMiss;  // a slab test alone does not reject
