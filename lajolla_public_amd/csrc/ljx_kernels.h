@@ -329,8 +329,11 @@ __device__ __forceinline__ int extend_finish(const DeviceScene &S, const PathBuf
     return P.volpath ? 0 : bucket;
 }
 
+// Triangle-only scenes: capped at 64 registers, eight waves per SIMD (the
+// LDS stacks allow eight), at the cost of a 16-byte spill; scenes with
+// spheres need the FP64 root solve's registers and keep their own count.
 template <bool STATS, bool SPH>
-__global__ __launch_bounds__(256) void k_extend(DeviceScene S, PathBufs P, int cur) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 1 : 8))) void k_extend(DeviceScene S, PathBufs P, int cur) {
     __shared__ int stack[kStackLds * 256];
     const uint32_t *q = cur ? P.extq1 : P.extq0;
     const int n = P.ctr[C_EXT + cur];
