@@ -459,9 +459,10 @@ __global__ __launch_bounds__(256) void k_vol_shade(DeviceScene S, PathBufs P, in
 // walk's first closest-hit query, over the same interval, would find
 // nothing).  Traversal only, so it runs at the any-hit kernel's occupancy
 // and with its lane refill; the walk's FP64 state would cut that to two
-// waves per SIMD.
+// waves per SIMD.  Triangle-only scenes cap it at eight waves per SIMD (64
+// registers, a 16-byte spill).
 template <bool STATS, bool SPH>
-__global__ __launch_bounds__(256) void k_vol_occlude(DeviceScene S, PathBufs P) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH ? 1 : 8))) void k_vol_occlude(DeviceScene S, PathBufs P) {
     __shared__ int stack[kStackLds * 256];
     const int n = P.ctr[C_SH];
     const TraceScene ts{S.qnodes, S.prims, S.spheres, S.slot_passthru};
