@@ -4,12 +4,14 @@
 // (src/intersection.cpp:7-85).  Node layout: a 4-wide BVH, the four child
 // boxes of a node as outward-rounded half offsets from a common corner, in
 // 80 bytes (ljx_device.h Bvh4QNode, quantised from the builders' FP32
-// Bvh4Node), loaded as five 16-byte vector loads; hit children are ordered near to far with a
-// five-comparator sorting network.  The traversal stack lives in LDS
+// Bvh4Node), loaded as five 16-byte vector loads.  For closest hits the hit
+// children are ordered near to far with a five-comparator sorting network;
+// any-hit rays take them in slot order.  The traversal stack lives in LDS
 // (kStackLds entries per lane, lane-interleaved so a wave's pushes hit 64
 // consecutive banks) and spills to a per-lane global area beyond that, sized
-// per scene from the tree's worst-case stack need.  Triangles use the watertight test of Woop, Benthin
-// and Wald (JCGT 2013) with an FP64 re-test of edge hits.
+// per scene from the tree's worst-case stack need.  Triangles use the
+// watertight test of Woop, Benthin and Wald (JCGT 2013) with an FP64 re-test
+// of edge hits.
 #pragma once
 
 #include "ljx_device.h"
@@ -240,28 +242,16 @@ struct Tracer {
     __device__ __forceinline__ bool step(const TraceScene &ts, int *stack, int stride, int *spill, int spill_stride,
                                          uint32_t &n_nodes, uint32_t &n_prims) {
         const float kMiss = __builtin_huge_valf();
-        // compare-exchange: by selects for closest hits (the branchy form
-        // compiles to exec-masked moves, two more operations each); any hit
-        // keeps the branchy form, which needs fewer registers there
+        // compare-exchange by selects (the branchy form compiles to
+        // exec-masked moves, two more operations each)
         auto cswap = [](float &ta, int &ca, float &tb, int &cb) {
-            if (ANY) {
-                if (tb < ta) {
-                    float t = ta;
-                    ta = tb;
-                    tb = t;
-                    int c = ca;
-                    ca = cb;
-                    cb = c;
-                }
-            } else {
-                const bool sw = tb < ta;
-                const float t0 = sw ? tb : ta, t1 = sw ? ta : tb;
-                const int k0 = sw ? cb : ca, k1 = sw ? ca : cb;
-                ta = t0;
-                tb = t1;
-                ca = k0;
-                cb = k1;
-            }
+            const bool sw = tb < ta;
+            const float t0 = sw ? tb : ta, t1 = sw ? ta : tb;
+            const int k0 = sw ? cb : ca, k1 = sw ? ca : cb;
+            ta = t0;
+            tb = t1;
+            ca = k0;
+            cb = k1;
         };
         if (node >= 0) {
             const Bvh4QNode *nd = ts.nodes + node;
@@ -281,27 +271,37 @@ struct Tracer {
             int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
             if (c2 == kNoChild) t2 = kMiss;  // empty slots carry inverted boxes, which
             if (c3 == kNoChild) t3 = kMiss;  // a slab test alone does not reject
-            // three comparators put the nearest child first; any hit needs no
-            // more (the others are pushed in whatever order, each if it was
-            // hit), closest hit orders all four with two more
-            cswap(t0, c0, t1, c1);
-            cswap(t2, c2, t3, c3);
-            cswap(t0, c0, t2, c2);
-            if (!ANY) {
-                cswap(t1, c1, t3, c3);
-                cswap(t1, c1, t2, c2);
-            }
-            if (t0 != kMiss) {
-                auto push = [&](int v) {
-                    if (sp < kStackLds) stack[sp * stride] = v;
-                    else spill[(sp - kStackLds) * spill_stride] = v;
-                    sp++;
-                };
+            auto push = [&](int v) {
+                if (sp < kStackLds) stack[sp * stride] = v;
+                else spill[(sp - kStackLds) * spill_stride] = v;
+                sp++;
+            };
+            if (ANY) {
+                // any hit: children in slot order, no sorting (an occluder
+                // ends the ray wherever it lies; sorting the children near
+                // to far measured slower)
                 if (t3 != kMiss) push(c3);
                 if (t2 != kMiss) push(c2);
                 if (t1 != kMiss) push(c1);
-                node = c0;
-                return false;
+                if (t0 != kMiss) {
+                    node = c0;
+                    return false;
+                }
+            } else {
+                // closest hit: five comparators order the four children,
+                // nearest taken next, the others pushed far to near
+                cswap(t0, c0, t1, c1);
+                cswap(t2, c2, t3, c3);
+                cswap(t0, c0, t2, c2);
+                cswap(t1, c1, t3, c3);
+                cswap(t1, c1, t2, c2);
+                if (t0 != kMiss) {
+                    if (t3 != kMiss) push(c3);
+                    if (t2 != kMiss) push(c2);
+                    if (t1 != kMiss) push(c1);
+                    node = c0;
+                    return false;
+                }
             }
         } else {
             const int code = ~node;
