@@ -296,9 +296,18 @@ struct Tracer {
                 cswap(t1, c1, t3, c3);
                 cswap(t1, c1, t2, c2);
                 if (t0 != kMiss) {
-                    if (t3 != kMiss) push(c3);
-                    if (t2 != kMiss) push(c2);
-                    if (t1 != kMiss) push(c1);
+                    if (sp <= kStackLds - 3) {
+                        // room for all three in LDS: no spill test per push
+                        int k = sp;
+                        if (t3 != kMiss) stack[(k++) * stride] = c3;
+                        if (t2 != kMiss) stack[(k++) * stride] = c2;
+                        if (t1 != kMiss) stack[(k++) * stride] = c1;
+                        sp = k;
+                    } else {
+                        if (t3 != kMiss) push(c3);
+                        if (t2 != kMiss) push(c2);
+                        if (t1 != kMiss) push(c1);
+                    }
                     node = c0;
                     return false;
                 }
