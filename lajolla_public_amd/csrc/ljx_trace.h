@@ -280,9 +280,20 @@ struct Tracer {
                 // any hit: children in slot order, no sorting (an occluder
                 // ends the ray wherever it lies; sorting the children near
                 // to far measured slower)
-                if (t3 != kMiss) push(c3);
-                if (t2 != kMiss) push(c2);
-                if (t1 != kMiss) push(c1);
+                // the volumetric occlusion test pushes without the per-push
+                // spill test while its LDS stack has room (faster there; the
+                // shadow-ray kernel measured slower with it)
+                if (OPAQUE && sp <= kStackLds - 3) {
+                    int k = sp;
+                    if (t3 != kMiss) stack[(k++) * stride] = c3;
+                    if (t2 != kMiss) stack[(k++) * stride] = c2;
+                    if (t1 != kMiss) stack[(k++) * stride] = c1;
+                    sp = k;
+                } else {
+                    if (t3 != kMiss) push(c3);
+                    if (t2 != kMiss) push(c2);
+                    if (t1 != kMiss) push(c1);
+                }
                 if (t0 != kMiss) {
                     node = c0;
                     return false;
